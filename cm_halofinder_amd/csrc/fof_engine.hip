@@ -1,0 +1,1199 @@
+// fof_engine.hip -- Friends-of-Friends linking pipeline for gfx950.
+//
+// Replaces the reference's cell lists + serial sweep + ring merge
+// (src_v2/FOF.c:44-161, Befriend :341-376) and computes what the host's
+// retrieval / Checkhalo / Output_Halos loops (src_v2/FOF.c:188-203,
+// :378-430, :797-847) derive from the ring.
+//
+// Pipeline (one launch per phase for all volumes of a batch):
+//   1 k_keys       reference cell of every particle (src_v2/FOF.c:79-83),
+//                  histogram, dense-cell list
+//   2 scan         cell start offsets
+//   3 k_scatter    counting sort -> float4 {x,y,z,index} in cell order
+//   4 k_subsort    cells with >= T particles: counting sort by sub-cell;
+//                  sub-cells have side <= l/sqrt(3), so each is a clique and
+//                  its particles start in one union-find set ("runs")
+//   5 k_link_sparse  one thread per particle of a sparse cell: own cell and
+//                  the 13 forward cells (src_v2/FOF.c:112-153)
+//   6 k_link_dense   one workgroup per dense cell: run pairs with early exit
+//   7 union-find finish: flatten, group minimum index, sizes
+//   8 labels, compaction of kept groups, stable radix sort by label
+//   9 ring in ascending index order, halo list, FP64 halo sums
+//
+// Pair rule (src_v2/FOF.c:351-357): dx = fl32(xi - xj) widened to double,
+// distsq = (dx^2 + dy^2) + dz^2 in FP64, linked iff !(distsq > linksq).
+// dx^2 is exact in FP64, so only the two additions round.  The FP32 test
+// d2 = fma(dz,dz,fma(dy,dy,dx*dx)) has relative error < 2e-7; outside the
+// band |d2/l2 - 1| >= band (>= 1e-6) it decides exactly as the FP64
+// expression would, inside it the FP64 expression is evaluated as written.
+#include <math.h>
+#include <string.h>
+
+#include "fof_internal.h"
+
+namespace fof {
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int vol_of(const uint32_t *__restrict__ vbeg, int nvol,
+                                      uint32_t p) {
+  int lo = 0, hi = nvol - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (vbeg[mid] <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Reference cell coordinate along one axis (src_v2/FOF.c:79-81), clamped.
+__device__ __forceinline__ int cell1(double x, double rmin_buff, double lcell, int n,
+                                     double *u) {
+  double t = (x - rmin_buff) / lcell;
+  *u = t;
+  int i = (int)floor(t);
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+struct CellPos {
+  int ix, iy, iz;   // reference cell
+  int fx, fy, fz;   // fine (sub-cell) coordinate = i*m + s
+};
+
+__device__ __forceinline__ int sub1(double u, int i, int m) {
+  int s = (int)floor((u - (double)i) * (double)m);
+  return s < 0 ? 0 : (s >= m ? m - 1 : s);
+}
+
+__device__ __forceinline__ CellPos cell_pos(const VolDev &V, float x, float y, float z) {
+  CellPos c;
+  double ux, uy, uz;
+  c.ix = cell1((double)x, V.rmin_buff[0], V.lcell[0], V.nx, &ux);
+  c.iy = cell1((double)y, V.rmin_buff[1], V.lcell[1], V.ny, &uy);
+  c.iz = cell1((double)z, V.rmin_buff[2], V.lcell[2], V.nz, &uz);
+  if (V.m[0] > 0) {
+    c.fx = c.ix * V.m[0] + sub1(ux, c.ix, V.m[0]);
+    c.fy = c.iy * V.m[1] + sub1(uy, c.iy, V.m[1]);
+    c.fz = c.iz * V.m[2] + sub1(uz, c.iz, V.m[2]);
+  } else {
+    c.fx = c.ix, c.fy = c.iy, c.fz = c.iz;
+  }
+  return c;
+}
+
+__device__ __forceinline__ uint32_t cell_id(const VolDev &V, int ix, int iy, int iz) {
+  return V.cbeg + ((uint32_t)iz * (uint32_t)V.ny + (uint32_t)iy) * (uint32_t)V.nx +
+         (uint32_t)ix;
+}
+
+// Squared linking length of a home cell: constant, or the variable rule
+// (same operation order as the oracle; built with -ffp-contract=off).
+__device__ __forceinline__ double cell_linksq(const VolDev &V, int ix, int iy, int iz) {
+  if (V.vntab == 0) return V.linksq;
+  double c0 = V.rmin_buff[0] + ((double)ix + 0.5) * V.lcell[0];
+  double c1 = V.rmin_buff[1] + ((double)iy + 0.5) * V.lcell[1];
+  double c2 = V.rmin_buff[2] + ((double)iz + 0.5) * V.lcell[2];
+  double e0 = c0 - V.vorigin[0], e1 = c1 - V.vorigin[1], e2 = c2 - V.vorigin[2];
+  double r2 = e0 * e0 + e1 * e1 + e2 * e2;
+  double top = (double)(V.vntab - 1);
+  double t = r2 / V.vr2max * top;
+  if (!(t < top)) return V.vtab[V.vntab - 1];
+  int k = (int)floor(t);
+  double f = t - (double)k;
+  return V.vtab[k] + f * (V.vtab[k + 1] - V.vtab[k]);
+}
+
+struct Rule {
+  double linksq;  // FP64 threshold (src_v2/FOF.c:357)
+  double prune2;  // squared sub-cell gap beyond which no pair can link
+  float lo, hi;   // FP32 decision thresholds
+};
+
+__device__ __forceinline__ Rule make_rule(const VolDev &V, int ix, int iy, int iz) {
+  Rule r;
+  if (V.vntab == 0) {
+    r.linksq = V.linksq;
+    r.lo = V.lo;
+    r.hi = V.hi;
+  } else {
+    r.linksq = cell_linksq(V, ix, iy, iz);
+    r.lo = (float)(r.linksq * (1.0 - (double)V.band));
+    r.hi = (float)(r.linksq * (1.0 + (double)V.band));
+  }
+  r.prune2 = r.linksq * (1.0 + 1e-4);
+  return r;
+}
+
+// The reference predicate with the FP32 fast path.
+__device__ __forceinline__ bool pair_linked(const float4 a, const float4 b, const Rule &r,
+                                            uint32_t &nre) {
+  const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  if (d2 < r.lo) return true;
+  if (d2 > r.hi) return false;
+  nre++;
+  const double ex = dx, ey = dy, ez = dz;
+  double ds = ex * ex + ey * ey;
+  ds = ds + ez * ez;
+  return !(ds > r.linksq);
+}
+
+// Union-find on cell-sorted indices; roots are hooked larger-under-smaller,
+// so every root is the smallest sorted index of its set and parent[x] <= x.
+// Loads are agent-coherent (skip the non-coherent per-CU cache); a stale
+// value can only be an older (larger) ancestor, so walks still terminate.
+__device__ __forceinline__ uint32_t ld_par(uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_par(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t uf_find(uint32_t *par, uint32_t x) {
+  uint32_t cur = ld_par(par + x);
+  if (cur == x) return x;
+  uint32_t prev = x, nxt;
+  while (cur > (nxt = ld_par(par + cur))) {
+    st_par(par + prev, nxt);  // path halving
+    prev = cur;
+    cur = nxt;
+  }
+  return cur;
+}
+
+__device__ __forceinline__ void uf_union(uint32_t *par, uint32_t a, uint32_t b) {
+  for (;;) {
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    if (a == b) return;
+    if (a > b) {
+      uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    uint32_t old = atomicCAS(par + b, b, a);
+    if (old == b) return;
+    b = old;
+  }
+}
+
+__device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t tests,
+                                             uint32_t rechecks) {
+  uint64_t t = tests, r = rechecks;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    t += __shfl_down(t, o, 64);
+    r += __shfl_down(r, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (t) atomicAdd(ctr + 0, (unsigned long long)t);
+    if (r) atomicAdd(ctr + 1, (unsigned long long)r);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1: keys + histogram
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_keys(const uint8_t *__restrict__ base, uint32_t stride,
+                                              uint32_t offpos, uint32_t N,
+                                              const VolDev *__restrict__ vols,
+                                              const uint32_t *__restrict__ vbeg, int nvol,
+                                              uint32_t *__restrict__ key,
+                                              uint32_t *__restrict__ cnt, uint32_t T,
+                                              uint32_t *__restrict__ dense,
+                                              uint32_t *__restrict__ ndense,
+                                              uint32_t *__restrict__ err) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += gridDim.x * blockDim.x) {
+    const int v = vol_of(vbeg, nvol, p);
+    const VolDev &V = vols[v];
+    const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
+    const float x = pp[0], y = pp[1], z = pp[2];
+    // src_v2/read_data.c:810-812 bounds; non-finite values are rejected too.
+    const bool ok = ((double)x >= V.rmin_buff[0]) && ((double)x <= V.rmax_buff[0]) &&
+                    ((double)y >= V.rmin_buff[1]) && ((double)y <= V.rmax_buff[1]) &&
+                    ((double)z >= V.rmin_buff[2]) && ((double)z <= V.rmax_buff[2]);
+    if (!ok) atomicOr(err, 1u);
+    double u;
+    const int ix = ok ? cell1((double)x, V.rmin_buff[0], V.lcell[0], V.nx, &u) : 0;
+    const int iy = ok ? cell1((double)y, V.rmin_buff[1], V.lcell[1], V.ny, &u) : 0;
+    const int iz = ok ? cell1((double)z, V.rmin_buff[2], V.lcell[2], V.nz, &u) : 0;
+    const uint32_t c = cell_id(V, ix, iy, iz);
+    key[p] = c;
+    const uint32_t old = atomicAdd(cnt + c, 1u);
+    if (old + 1 == T && V.m[0] > 0) dense[atomicAdd(ndense, 1u)] = c;
+  }
+}
+
+// 3: counting-sort scatter.  cnt[c] counts down to 0.
+__global__ __launch_bounds__(256) void k_scatter(const uint8_t *__restrict__ base,
+                                                 uint32_t stride, uint32_t offpos, uint32_t N,
+                                                 const uint32_t *__restrict__ key,
+                                                 const uint32_t *__restrict__ start,
+                                                 uint32_t *__restrict__ cnt,
+                                                 float4 *__restrict__ pos4) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += gridDim.x * blockDim.x) {
+    const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
+    const uint32_t c = key[p];
+    const uint32_t slot = start[c] + atomicSub(cnt + c, 1u) - 1u;
+    pos4[slot] = make_float4(pp[0], pp[1], pp[2], __uint_as_float(p));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_init_uf(uint32_t N, uint32_t *__restrict__ par,
+                                                 uint32_t *__restrict__ run_end) {
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += gridDim.x * blockDim.x) {
+    par[x] = x;
+    run_end[x] = x + 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 4: sub-cell counting sort of dense cells; runs become cliques.
+// ---------------------------------------------------------------------------
+constexpr int kMaxBins = kMaxSub * kMaxSub * kMaxSub;  // 4096
+
+__device__ __forceinline__ int sub_bin(const VolDev &V, const float4 p) {
+  double ux, uy, uz;
+  const int ix = cell1((double)p.x, V.rmin_buff[0], V.lcell[0], V.nx, &ux);
+  const int iy = cell1((double)p.y, V.rmin_buff[1], V.lcell[1], V.ny, &uy);
+  const int iz = cell1((double)p.z, V.rmin_buff[2], V.lcell[2], V.nz, &uz);
+  return (sub1(uz, iz, V.m[2]) * V.m[1] + sub1(uy, iy, V.m[1])) * V.m[0] + sub1(ux, ix, V.m[0]);
+}
+
+__global__ __launch_bounds__(256) void k_subsort(const uint32_t *__restrict__ dense,
+                                                 const uint32_t *__restrict__ ndense,
+                                                 const uint32_t *__restrict__ start,
+                                                 const VolDev *__restrict__ vols,
+                                                 const uint32_t *__restrict__ vbeg, int nvol,
+                                                 float4 *__restrict__ pos4,
+                                                 float4 *__restrict__ tmp4,
+                                                 uint32_t *__restrict__ par,
+                                                 uint32_t *__restrict__ run_end) {
+  __shared__ uint32_t bstart[kMaxBins];
+  __shared__ uint32_t bcur[kMaxBins];
+  __shared__ uint32_t ws[8];
+  const uint32_t nd = *ndense;
+  for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
+    const uint32_t c = dense[d];
+    const uint32_t s = start[c], e = start[c + 1], n = e - s;
+    const VolDev &V = vols[vol_of(vbeg, nvol, s)];
+    const int nbins = V.m[0] * V.m[1] * V.m[2];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) bcur[b] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+      atomicAdd(&bcur[sub_bin(V, pos4[s + i])], 1u);
+    __syncthreads();
+    // exclusive scan of bcur[0..nbins) -> bstart (blocked, 16 bins/thread)
+    {
+      const int b0 = threadIdx.x * 16;
+      uint32_t loc = 0;
+      for (int q = 0; q < 16; q++)
+        if (b0 + q < nbins) loc += bcur[b0 + q];
+      uint32_t inc = loc;
+      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+      }
+      if (lane == 63) ws[w] = inc;
+      __syncthreads();
+      uint32_t off = 0;
+      for (int q = 0; q < w; q++) off += ws[q];
+      uint32_t pre = off + inc - loc;
+      for (int q = 0; q < 16; q++)
+        if (b0 + q < nbins) {
+          uint32_t cc = bcur[b0 + q];
+          bstart[b0 + q] = pre;
+          bcur[b0 + q] = pre;
+          pre += cc;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const float4 p = pos4[s + i];
+      const uint32_t slot = atomicAdd(&bcur[sub_bin(V, p)], 1u);
+      tmp4[s + slot] = p;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const float4 p = tmp4[s + i];
+      const int b = sub_bin(V, p);
+      pos4[s + i] = p;
+      par[s + i] = s + bstart[b];
+      run_end[s + i] = s + bcur[b];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5: sparse cells -- one thread per particle.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *__restrict__ pos4,
+                                                     const uint32_t *__restrict__ start,
+                                                     const uint32_t *__restrict__ run_end,
+                                                     uint32_t *__restrict__ par,
+                                                     const VolDev *__restrict__ vols,
+                                                     const uint32_t *__restrict__ vbeg,
+                                                     int nvol, uint32_t T,
+                                                     unsigned long long *__restrict__ ctr) {
+  uint32_t tests = 0, rechecks = 0;
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += gridDim.x * blockDim.x) {
+    const float4 px = pos4[x];
+    const VolDev &V = vols[vol_of(vbeg, nvol, x)];
+    const CellPos cp = cell_pos(V, px.x, px.y, px.z);
+    const uint32_t c = cell_id(V, cp.ix, cp.iy, cp.iz);
+    const uint32_t s = start[c], e = start[c + 1];
+    if (V.m[0] > 0 && e - s >= T) continue;  // dense: k_link_dense
+    const Rule R = make_rule(V, cp.ix, cp.iy, cp.iz);
+    const int nx = V.nx, ny = V.ny, nz = V.nz;
+    const uint32_t xl = (cp.ix > 0) ? 1u : 0u, xr = (cp.ix + 1 < nx) ? 1u : 0u;
+    // Up to five contiguous ranges cover the own cell (after x) and the 13
+    // forward cells: (ix+1,iy,iz); row (iy+1,iz); rows (iy-1..iy+1, iz+1).
+    uint32_t r0[5], r1[5];
+    int nr = 0;
+    r0[nr] = x + 1;
+    r1[nr] = start[c + 1 + xr];
+    nr++;
+    if (cp.iy + 1 < ny) {
+      const uint32_t cc = c + (uint32_t)nx;
+      r0[nr] = start[cc - xl];
+      r1[nr] = start[cc + xr + 1];
+      nr++;
+    }
+    if (cp.iz + 1 < nz) {
+      for (int dy = -1; dy <= 1; dy++) {
+        if (cp.iy + dy < 0 || cp.iy + dy >= ny) continue;
+        const uint32_t cc = c + (uint32_t)nx * (uint32_t)ny + (uint32_t)(dy * nx);
+        r0[nr] = start[cc - xl];
+        r1[nr] = start[cc + xr + 1];
+        nr++;
+      }
+    }
+    for (int r = 0; r < nr; r++) {
+      uint32_t j = r0[r];
+      const uint32_t je = r1[r];
+      while (j < je) {
+        const uint32_t re = run_end[j];
+        if (re == j + 1) {
+          tests++;
+          if (pair_linked(px, pos4[j], R, rechecks)) uf_union(par, x, j);
+          j++;
+          continue;
+        }
+        // a clique run of a dense cell: skip it when out of reach or joined
+        const float4 pj = pos4[j];
+        const CellPos cj = cell_pos(V, pj.x, pj.y, pj.z);
+        const int gx = max(abs(cj.fx - cp.fx) - 1, 0);
+        const int gy = max(abs(cj.fy - cp.fy) - 1, 0);
+        const int gz = max(abs(cj.fz - cp.fz) - 1, 0);
+        const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
+                          (double)(gz * gz) * V.s2[2];
+        if (!(g2 > R.prune2) && uf_find(par, x) != uf_find(par, j)) {
+          for (uint32_t k = j; k < re; k++) {
+            tests++;
+            if (pair_linked(px, pos4[k], R, rechecks)) {
+              uf_union(par, x, k);
+              break;
+            }
+          }
+        }
+        j = re;
+      }
+    }
+  }
+  add_counters(ctr, tests, rechecks);
+}
+
+// ---------------------------------------------------------------------------
+// 6: dense cells -- one workgroup per cell, run pairs with early exit.
+// ---------------------------------------------------------------------------
+constexpr int kMaxRuns = 2048;
+
+__device__ __forceinline__ uint32_t pack_rel(int a, int b, int c) {
+  return ((uint32_t)(a & 255)) | ((uint32_t)(b & 255) << 8) | ((uint32_t)(c & 255) << 16);
+}
+__device__ __forceinline__ int rel_x(uint32_t r) { return (int)(int8_t)(r & 255); }
+__device__ __forceinline__ int rel_y(uint32_t r) { return (int)(int8_t)((r >> 8) & 255); }
+__device__ __forceinline__ int rel_z(uint32_t r) { return (int)(int8_t)((r >> 16) & 255); }
+
+// Collect the runs of sorted range [s,e) into LDS (head, end, fine coordinate
+// relative to the home cell's corner).  Keeps runs whose relative coordinate
+// passes the face filter: for each axis with window w[d] = (lo,hi), only
+// runs with lo <= rel <= hi.
+__device__ int collect_runs(const float4 *__restrict__ pos4,
+                            const uint32_t *__restrict__ run_end, uint32_t s, uint32_t e,
+                            const VolDev &V, int ox, int oy, int oz, int wlo0, int whi0,
+                            int wlo1, int whi1, int wlo2, int whi2, uint32_t *head,
+                            uint32_t *end, uint32_t *rel, uint32_t *cnt) {
+  if (threadIdx.x == 0) *cnt = 0;
+  __syncthreads();
+  for (uint32_t j = s + threadIdx.x; j < e; j += blockDim.x) {
+    const uint32_t re = run_end[j];
+    const bool isstart = (j == s) || (run_end[j - 1] != re);
+    if (!isstart) continue;
+    const float4 pj = pos4[j];
+    const CellPos c = cell_pos(V, pj.x, pj.y, pj.z);
+    const int rx = c.fx - ox, ry = c.fy - oy, rz = c.fz - oz;
+    if (rx < wlo0 || rx > whi0 || ry < wlo1 || ry > whi1 || rz < wlo2 || rz > whi2) continue;
+    const uint32_t k = atomicAdd(cnt, 1u);
+    if (k < (uint32_t)kMaxRuns) {
+      head[k] = j;
+      end[k] = re;
+      rel[k] = pack_rel(rx, ry, rz);
+    }
+  }
+  __syncthreads();
+  const uint32_t n = *cnt;
+  __syncthreads();
+  return (int)n;
+}
+
+__global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__ dense,
+                                                    const uint32_t *__restrict__ ndense,
+                                                    const float4 *__restrict__ pos4,
+                                                    const uint32_t *__restrict__ start,
+                                                    const uint32_t *__restrict__ run_end,
+                                                    uint32_t *__restrict__ par,
+                                                    const VolDev *__restrict__ vols,
+                                                    const uint32_t *__restrict__ vbeg,
+                                                    int nvol,
+                                                    unsigned long long *__restrict__ ctr,
+                                                    uint32_t *__restrict__ overflow) {
+  __shared__ uint32_t ahead[kMaxRuns], aend[kMaxRuns], arel[kMaxRuns];
+  __shared__ uint32_t bhead[kMaxRuns], bend[kMaxRuns], brel[kMaxRuns];
+  __shared__ uint32_t cnt;
+  uint32_t tests = 0, rechecks = 0;
+  const uint32_t nd = *ndense;
+  for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
+    const uint32_t c = dense[d];
+    const uint32_t sA = start[c], eA = start[c + 1];
+    const VolDev &V = vols[vol_of(vbeg, nvol, sA)];
+    const uint32_t lc = c - V.cbeg;
+    const int ix = (int)(lc % (uint32_t)V.nx);
+    const int iy = (int)((lc / (uint32_t)V.nx) % (uint32_t)V.ny);
+    const int iz = (int)(lc / ((uint32_t)V.nx * (uint32_t)V.ny));
+    const Rule R = make_rule(V, ix, iy, iz);
+    const int mx = V.m[0], my = V.m[1], mz = V.m[2];
+    const int ox = ix * mx, oy = iy * my, oz = iz * mz;
+    // 14 home-cell pairs: (A,A) and the 13 forward cells.
+    for (int q = 0; q < 14; q++) {
+      int dx = 0, dy = 0, dz = 0;
+      if (q > 0) {
+        // enumerate forward offsets in the order of src_v2/FOF.c:122-153
+        int k = q - 1;  // 0..12
+        if (k == 0) dx = 1;
+        else if (k < 4) dy = 1, dx = k - 2;
+        else {
+          dz = 1;
+          dy = (k - 4) / 3 - 1;
+          dx = (k - 4) % 3 - 1;
+        }
+      }
+      const int bx = ix + dx, by = iy + dy, bz = iz + dz;
+      if (bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz >= V.nz) continue;
+      const uint32_t cb = cell_id(V, bx, by, bz);
+      const uint32_t sB = start[cb], eB = start[cb + 1];
+      if (sB == eB) continue;
+      // Face filter: only runs within two sub-cells of the shared boundary.
+      int alo[3] = {-1000, -1000, -1000}, ahi[3] = {1000, 1000, 1000};
+      int blo[3] = {-1000, -1000, -1000}, bhi[3] = {1000, 1000, 1000};
+      const int dd[3] = {dx, dy, dz}, mm[3] = {mx, my, mz};
+      for (int a = 0; a < 3; a++) {
+        // largest sub-cell offset whose gap can still be within reach
+        int reach = (int)floor(sqrt(R.prune2 / V.s2[a])) + 1;
+        reach = min(reach, mm[a]);
+        if (dd[a] == 1) {
+          alo[a] = mm[a] - reach;
+          blo[a] = mm[a];
+          bhi[a] = mm[a] - 1 + reach;
+        } else if (dd[a] == -1) {
+          ahi[a] = reach - 1;
+          blo[a] = -reach;
+          bhi[a] = -1;
+        }
+      }
+      const int na = collect_runs(pos4, run_end, sA, eA, V, ox, oy, oz, alo[0], ahi[0], alo[1],
+                                  ahi[1], alo[2], ahi[2], ahead, aend, arel, &cnt);
+      int nb = 0;
+      if (q > 0)
+        nb = collect_runs(pos4, run_end, sB, eB, V, ox, oy, oz, blo[0], bhi[0], blo[1],
+                          bhi[1], blo[2], bhi[2], bhead, bend, brel, &cnt);
+      if (na > kMaxRuns || nb > kMaxRuns) {
+        if (threadIdx.x == 0) atomicOr(overflow, 1u);
+        continue;
+      }
+      if (na == 0 || (q > 0 && nb == 0)) continue;
+      const uint32_t *h2 = (q == 0) ? ahead : bhead;
+      const uint32_t *e2 = (q == 0) ? aend : bend;
+      const uint32_t *r2 = (q == 0) ? arel : brel;
+      const int n2 = (q == 0) ? na : nb;
+      for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
+        const uint32_t ra = arel[ia];
+        const int ax = rel_x(ra), ay = rel_y(ra), az = rel_z(ra);
+        const uint32_t ha = ahead[ia], ea = aend[ia];
+        for (int ib = (q == 0) ? ia + 1 : 0; ib < n2; ib++) {
+          const uint32_t rb = r2[ib];
+          const int gx = max(abs(rel_x(rb) - ax) - 1, 0);
+          const int gy = max(abs(rel_y(rb) - ay) - 1, 0);
+          const int gz = max(abs(rel_z(rb) - az) - 1, 0);
+          const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
+                            (double)(gz * gz) * V.s2[2];
+          if (g2 > R.prune2) continue;
+          const uint32_t hb = h2[ib];
+          if (uf_find(par, ha) == uf_find(par, hb)) continue;
+          const uint32_t eb = e2[ib];
+          bool linked = false;
+          for (uint32_t i = ha; i < ea && !linked; i++) {
+            const float4 pi = pos4[i];
+            for (uint32_t j = hb; j < eb; j++) {
+              tests++;
+              if (pair_linked(pi, pos4[j], R, rechecks)) {
+                linked = true;
+                break;
+              }
+            }
+          }
+          if (linked) uf_union(par, ha, hb);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  add_counters(ctr, tests, rechecks);
+}
+
+// ---------------------------------------------------------------------------
+// 7: union-find finish
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_flatten(uint32_t N, uint32_t *__restrict__ par,
+                                                 const float4 *__restrict__ pos4,
+                                                 uint32_t *__restrict__ minorig,
+                                                 uint32_t *__restrict__ gsize,
+                                                 unsigned long long *__restrict__ ctr) {
+  uint32_t roots = 0;
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += gridDim.x * blockDim.x) {
+    uint32_t r = x, p = par[x];
+    while (p != r) {
+      r = p;
+      p = par[r];
+    }
+    par[x] = r;
+    minorig[x] = __float_as_uint(pos4[x].w);
+    gsize[x] = 1;
+    roots += (r == x);
+  }
+  uint64_t t = roots;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+  if ((threadIdx.x & 63) == 0 && t) atomicAdd(ctr + 2, (unsigned long long)t);
+}
+
+__global__ __launch_bounds__(256) void k_gather(uint32_t N, const uint32_t *__restrict__ par,
+                                                const float4 *__restrict__ pos4,
+                                                uint32_t *__restrict__ minorig,
+                                                uint32_t *__restrict__ gsize) {
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += gridDim.x * blockDim.x) {
+    const uint32_t r = par[x];
+    if (r != x) {
+      atomicMin(minorig + r, __float_as_uint(pos4[x].w));
+      atomicAdd(gsize + r, 1u);
+    }
+  }
+}
+
+// label[orig] = group minimum index (global), with kFlag when the group is
+// kept for the ring sort (size >= 2, or size >= nphalomin).  The head's
+// entry of hsize receives the group size.
+__global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__restrict__ par,
+                                               const float4 *__restrict__ pos4,
+                                               const uint32_t *__restrict__ minorig,
+                                               const uint32_t *__restrict__ gsize,
+                                               const VolDev *__restrict__ vols,
+                                               const uint32_t *__restrict__ vbeg, int nvol,
+                                               uint32_t *__restrict__ label,
+                                               uint32_t *__restrict__ hsize) {
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += gridDim.x * blockDim.x) {
+    const uint32_t r = par[x];
+    const uint32_t lab = minorig[r], sz = gsize[r];
+    const uint32_t o = __float_as_uint(pos4[x].w);
+    const VolDev &V = vols[vol_of(vbeg, nvol, x)];
+    const bool keep = (sz >= 2u) || !((double)sz < V.nphalomin);
+    label[o] = lab | (keep ? kFlag : 0u);
+    if (o == lab) hsize[lab] = sz;
+  }
+}
+
+// Outputs for every particle: label_out (volume-local) and, for particles not
+// in the sorted list, the trivial ring entry next = self (1-based).
+__global__ __launch_bounds__(256) void k_out_init(uint32_t N, const uint32_t *__restrict__ label,
+                                                  const uint32_t *__restrict__ vbeg, int nvol,
+                                                  uint32_t *__restrict__ label_out,
+                                                  uint32_t *__restrict__ next_out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t l = label[i];
+    const uint32_t b = vbeg[vol_of(vbeg, nvol, i)];
+    if (label_out) label_out[i] = (l & ~kFlag) - b;
+    if (next_out && !(l & kFlag)) next_out[i] = i - b + 1u;
+  }
+}
+
+// Ring in ascending index order over the label-sorted list, and halo starts.
+__global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
+                                              const uint32_t *__restrict__ skey,
+                                              const uint32_t *__restrict__ sval,
+                                              const uint32_t *__restrict__ hsize,
+                                              const VolDev *__restrict__ vols,
+                                              const uint32_t *__restrict__ vbeg, int nvol,
+                                              uint32_t *__restrict__ next_out,
+                                              uint32_t *__restrict__ hflag) {
+  const uint32_t M = *dM;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < M;
+       k += gridDim.x * blockDim.x) {
+    const uint32_t lab = skey[k], i = sval[k];
+    const int v = vol_of(vbeg, nvol, i);
+    const uint32_t b = vbeg[v];
+    if (next_out) {
+      const uint32_t nx = (k + 1 < M && skey[k + 1] == lab) ? sval[k + 1] : lab;
+      next_out[i] = nx - b + 1u;
+    }
+    const bool gstart = (k == 0) || (skey[k - 1] != lab);
+    hflag[k] = (gstart && !((double)hsize[lab] < vols[v].nphalomin)) ? kFlag : 0u;
+  }
+}
+
+// Halo sums: FP64 accumulation of the float Pos / Vel of the members in
+// ascending index order from the head (the order the host walks the ring
+// returned in next_out), so the host's Checkhalo / Output_Halos loops give
+// the same bits.  Loads are issued 16 ahead of the ordered accumulation.
+__global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ dH,
+                                                   const uint32_t *__restrict__ hk,
+                                                   const uint32_t *__restrict__ skey,
+                                                   const uint32_t *__restrict__ sval,
+                                                   const uint32_t *__restrict__ hsize,
+                                                   const uint8_t *__restrict__ base,
+                                                   uint32_t stride, uint32_t offpos,
+                                                   uint32_t offvel,
+                                                   const VolDev *__restrict__ vols,
+                                                   const uint32_t *__restrict__ vbeg, int nvol,
+                                                   fof_halo_v1 *__restrict__ out,
+                                                   uint32_t *__restrict__ hhead,
+                                                   unsigned long long *__restrict__ ctr) {
+  const uint32_t H = *dH;
+  uint32_t owned_cnt = 0;
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H;
+       h += gridDim.x * blockDim.x) {
+    const uint32_t k0 = hk[h];
+    const uint32_t lab = skey[k0];
+    const uint32_t n = hsize[lab];
+    const int v = vol_of(vbeg, nvol, lab);
+    const VolDev &V = vols[v];
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    constexpr int kAhead = 16;
+    for (uint32_t k = 0; k < n; k += kAhead) {
+      float a[kAhead][6];
+      const uint32_t m = min((uint32_t)kAhead, n - k);
+#pragma unroll
+      for (int q = 0; q < kAhead; q++) {
+        if ((uint32_t)q < m) {
+          const uint8_t *rec = base + (uint64_t)sval[k0 + k + q] * stride;
+          const float *pp = reinterpret_cast<const float *>(rec + offpos);
+          const float *vv = reinterpret_cast<const float *>(rec + offvel);
+          a[q][0] = pp[0], a[q][1] = pp[1], a[q][2] = pp[2];
+          a[q][3] = vv[0], a[q][4] = vv[1], a[q][5] = vv[2];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kAhead; q++) {
+        if ((uint32_t)q < m) {
+#pragma unroll
+          for (int t = 0; t < 6; t++) s[t] += (double)a[q][t];
+        }
+      }
+    }
+    fof_halo_v1 H1;
+    H1.head = lab - vbeg[v];
+    H1.n = n;
+    // src_v2/FOF.c:394-401
+    const double cx = s[0] / (double)n, cy = s[1] / (double)n, cz = s[2] / (double)n;
+    const bool own = !((cx < V.rmin[0]) || (cx >= V.rmax[0]) || (cy < V.rmin[1]) ||
+                       (cy >= V.rmax[1]) || (cz < V.rmin[2]) || (cz >= V.rmax[2]));
+    uint32_t warn = 0;
+    if (own) {
+      // src_v2/FOF.c:403-422
+      for (uint32_t k = 0; k < n; k++) {
+        const float *pp =
+            reinterpret_cast<const float *>(base + (uint64_t)sval[k0 + k] * stride + offpos);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const double p = pp[d];
+          if ((p - V.rmin_buff[d] < V.linklength) || (V.rmax_buff[d] - p < V.linklength))
+            warn |= 1u << d;
+        }
+      }
+      owned_cnt++;
+    }
+    H1.owned = own ? 1u : 0u;
+    H1.edge_warn = warn;
+    for (int t = 0; t < 3; t++) {
+      H1.sum_pos[t] = s[t];
+      H1.sum_vel[t] = s[3 + t];
+    }
+    out[h] = H1;
+    hhead[h] = lab;
+  }
+  uint64_t t = owned_cnt;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+  if ((threadIdx.x & 63) == 0 && t) atomicAdd(ctr + 3, (unsigned long long)t);
+}
+
+// halo_offsets[v] = first halo whose global head >= vbeg[v].
+__global__ void k_halo_offsets(const uint32_t *__restrict__ dH,
+                               const uint32_t *__restrict__ hhead,
+                               const uint32_t *__restrict__ vbeg, int nvol,
+                               uint64_t *__restrict__ off) {
+  const uint32_t H = *dH;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v <= nvol; v += gridDim.x * blockDim.x) {
+    const uint32_t key = vbeg[v];
+    uint32_t lo = 0, hi = H;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (hhead[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    off[v] = (v == nvol) ? H : lo;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+static inline uint32_t grid_for(uint64_t n, uint32_t cap = 65535u * 4u) {
+  uint64_t g = (n + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (uint32_t)g;
+}
+
+static int bits_for(uint64_t n) {
+  int b = 1;
+  while (b < 32 && (1ull << b) < n) b++;
+  return b;
+}
+
+// Host-side volume preparation: grid and thresholds, validated.
+struct HostVol {
+  VolDev d;
+  uint64_t ncell;
+};
+
+int prepare_volume(const fof_volume_v1 &in, double band, HostVol &hv, std::vector<const fof_varlink_v1 *> &vl) {
+  const fof_domain_v1 &D = in.domain;
+  VolDev &V = hv.d;
+  memset(&V, 0, sizeof(V));
+  for (int k = 0; k < 3; k++) {
+    V.rmin_buff[k] = D.rmin_buff[k];
+    V.rmax_buff[k] = D.rmax_buff[k];
+    V.rmin[k] = D.rmin[k];
+    V.rmax[k] = D.rmax[k];
+    V.lcell[k] = D.lcell[k];
+    if (!(D.lcell[k] > 0) || !std::isfinite(D.lcell[k]) || !(D.rmax_buff[k] >= D.rmin_buff[k]))
+      return FOF_E_ARG;
+  }
+  if (!(D.linklength >= 0) || !std::isfinite(D.linklength)) return FOF_E_ARG;
+  V.linklength = D.linklength;
+  V.linksq = D.linklength * D.linklength;  // src_v2/FOF.c:60
+  V.nphalomin = D.nphalomin;
+  V.band = (float)band;
+  V.lo = (float)(V.linksq * (1.0 - band));
+  V.hi = (float)(V.linksq * (1.0 + band));
+  // src_v2/FOF.c:63-65
+  double e[3];
+  for (int k = 0; k < 3; k++) e[k] = ceil((D.rmax_buff[k] - D.rmin_buff[k]) / D.lcell[k]) + 1.0;
+  if (e[0] * e[1] * e[2] > 2147483000.0) return FOF_E_RANGE;
+  V.nx = (int)e[0], V.ny = (int)e[1], V.nz = (int)e[2];
+  hv.ncell = (uint64_t)V.nx * V.ny * V.nz;
+  // Smallest linking length that any home cell can use.
+  double lmin2 = V.linksq;
+  if (D.varlink) {
+    const fof_varlink_v1 *L = D.varlink;
+    if (L->ntab < 2 || !L->l2tab || !(L->r2max > 0)) return FOF_E_ARG;
+    vl.push_back(L);
+    V.vntab = L->ntab;
+    V.vr2max = L->r2max;
+    for (int k = 0; k < 3; k++) V.vorigin[k] = L->origin[k];
+    lmin2 = -1;  // filled by the caller once the table is on the host
+  }
+  // Sub-cells: side <= l_min (1 - 1e-6) / sqrt(3) so that any two particles
+  // of one sub-cell link (see file header).  Disabled when more than
+  // kMaxSub per axis would be needed, or l = 0.
+  V.m[0] = V.m[1] = V.m[2] = 0;
+  if (lmin2 > 0) {
+    const double s = sqrt(lmin2) * (1.0 - 1e-6) / sqrt(3.0);
+    int m[3];
+    bool ok = true;
+    for (int k = 0; k < 3; k++) {
+      double q = ceil(D.lcell[k] / s);
+      if (q < 1) q = 1;
+      if (q > 12) ok = false;
+      m[k] = (int)q;
+    }
+    if (ok)
+      for (int k = 0; k < 3; k++) {
+        V.m[k] = m[k];
+        const double side = D.lcell[k] / m[k];
+        V.s2[k] = side * side;
+      }
+  }
+  return 0;
+}
+
+}  // namespace fof
+
+using namespace fof;
+
+extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1 *vols,
+                              uint32_t nvol, const fof_opts_v1 *opts, uint32_t *next_out,
+                              uint32_t *label_out, fof_halo_v1 *halos, uint64_t halo_cap,
+                              uint64_t *halo_offsets, uint64_t *nhalos_out,
+                              fof_stats_v1 *stats);
+
+extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1 *vols,
+                              uint32_t nvol, const fof_opts_v1 *opts, uint32_t *next_out,
+                              uint32_t *label_out, fof_halo_v1 *halos, uint64_t halo_cap,
+                              uint64_t *halo_offsets, uint64_t *nhalos_out,
+                              fof_stats_v1 *stats) {
+  EngineState &S = ctx->st;
+  Work &W = S.w;
+  hipStream_t st = S.stream;
+  const uint32_t flags = opts ? opts->flags : 0u;
+  double band = (opts && opts->band > 0) ? opts->band : 1e-5;
+  if (!(band >= 1e-6 && band <= 0.1)) return FOF_E_ARG;
+  const uint32_t T = (opts && opts->dense_threshold) ? opts->dense_threshold : 12u;
+  const bool out_dev = (flags & FOF_OPT_OUT_DEVICE) != 0;
+  const bool want_ring = next_out && !(flags & FOF_OPT_NO_RING);
+  const bool want_halos = !(flags & FOF_OPT_NO_HALOS);
+  const bool timing = (flags & FOF_OPT_TIMING) != 0;
+
+  // ---- volumes ----
+  uint64_t N = 0, C = 0;
+  std::vector<HostVol> hv(nvol);
+  std::vector<const fof_varlink_v1 *> vls;
+  std::vector<uint32_t> vbeg(nvol + 1);
+  for (uint32_t v = 0; v < nvol; v++) {
+    if (vols[v].offset != N) return FOF_E_ARG;  // volumes must be back to back
+    int rc = prepare_volume(vols[v], band, hv[v], vls);
+    if (rc) return rc;
+    vbeg[v] = (uint32_t)N;
+    hv[v].d.pbeg = (uint32_t)N;
+    hv[v].d.cbeg = (uint32_t)C;
+    N += vols[v].n;
+    C += hv[v].ncell;
+    hv[v].d.pend = (uint32_t)N;
+    hv[v].d.ncell = (uint32_t)hv[v].ncell;
+    if (N >= 0x7FFFFFFFull || C >= 0x7FFFFFF0ull) return FOF_E_RANGE;
+  }
+  vbeg[nvol] = (uint32_t)N;
+  if (N != P->n) return FOF_E_ARG;
+
+  // Variable linking-length tables are copied to the device back to back.
+  size_t vt_entries = 0;
+  for (uint32_t v = 0; v < nvol; v++)
+    if (vols[v].domain.varlink) vt_entries += vols[v].domain.varlink->ntab;
+  std::vector<double> vt_host(vt_entries);
+  {
+    size_t o = 0;
+    for (uint32_t v = 0; v < nvol; v++) {
+      const fof_varlink_v1 *L = vols[v].domain.varlink;
+      if (!L) continue;
+      if (L->location == FOF_MEM_DEVICE)
+        FOF_HIP(hipMemcpy(&vt_host[o], L->l2tab, L->ntab * sizeof(double), hipMemcpyDeviceToHost));
+      else
+        memcpy(&vt_host[o], L->l2tab, L->ntab * sizeof(double));
+      double lmin2 = vt_host[o];
+      for (uint32_t k = 0; k < L->ntab; k++) {
+        if (!(vt_host[o + k] >= 0) || !std::isfinite(vt_host[o + k])) return FOF_E_ARG;
+        lmin2 = std::min(lmin2, vt_host[o + k]);
+      }
+      VolDev &V = hv[v].d;
+      if (lmin2 > 0) {
+        const double s = sqrt(lmin2) * (1.0 - 1e-6) / sqrt(3.0);
+        bool ok = true;
+        int m[3];
+        for (int k = 0; k < 3; k++) {
+          double q = ceil(V.lcell[k] / s);
+          if (q < 1) q = 1;
+          if (q > 12) ok = false;
+          m[k] = (int)q;
+        }
+        if (ok)
+          for (int k = 0; k < 3; k++) {
+            V.m[k] = m[k];
+            const double side = V.lcell[k] / m[k];
+            V.s2[k] = side * side;
+          }
+      }
+      o += L->ntab;
+    }
+  }
+
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->version = FOF_ABI_VERSION;
+    stats->nvol = nvol;
+    stats->n = N;
+    stats->ncells = C;
+  }
+  if (halo_offsets && !want_halos)
+    for (uint32_t v = 0; v <= nvol; v++) halo_offsets[v] = 0;
+  if (N == 0) {
+    if (nhalos_out) *nhalos_out = 0;
+    if (halo_offsets)
+      for (uint32_t v = 0; v <= nvol; v++) halo_offsets[v] = 0;
+    return 0;
+  }
+
+  // ---- workspace ----
+  const uint64_t ndense_cap = N / (T ? T : 1) + 2;
+  int rc = 0;
+  if ((rc = ensure(W.pos4, N * 16))) return rc;
+  if ((rc = ensure(W.key, N * 4 + 64))) return rc;
+  if ((rc = ensure(W.cnt, C * 4 + 64))) return rc;
+  if ((rc = ensure(W.start, (C + 2) * 4 + 64))) return rc;
+  if ((rc = ensure(W.par, N * 4 + 64))) return rc;
+  if ((rc = ensure(W.runend, N * 4 + 64))) return rc;
+  if ((rc = ensure(W.gsize, N * 4 + 64))) return rc;
+  if ((rc = ensure(W.label, N * 4 + 64))) return rc;
+  const uint64_t q4 = (N + 63) / 64 * 64;  // u32 entries per sort array
+  if ((rc = ensure(W.sortbuf, 16 * q4 + 256))) return rc;
+  if ((rc = ensure(W.dense, ndense_cap * 4 + 64))) return rc;
+  size_t tmpb = std::max(std::max(scan_tmp_bytes(C + 1), sort_tmp_bytes(N)), compact_tmp_bytes(N));
+  if ((rc = ensure(W.tmp, tmpb))) return rc;
+  if ((rc = ensure(W.vols, nvol * sizeof(VolDev) + vt_entries * sizeof(double) + 256))) return rc;
+  if ((rc = ensure(W.vbeg, (nvol + 1) * 4 + 64))) return rc;
+  if ((rc = ensure(W.small, 256))) return rc;
+  if ((rc = ensure(W.hhead, N * 4 + 64))) return rc;
+  if ((rc = ensure(W.hoff, (nvol + 1) * 8 + 64))) return rc;
+
+  // device copy of the particle array when it lives on the host
+  const uint8_t *dbase = static_cast<const uint8_t *>(P->base);
+  if (P->location != FOF_MEM_DEVICE) {
+    if ((rc = ensure(W.input, N * P->stride))) return rc;
+    FOF_HIP(hipMemcpyAsync(W.input.ptr, P->base, N * P->stride, hipMemcpyHostToDevice, st));
+    dbase = static_cast<const uint8_t *>(W.input.ptr);
+  }
+  uint32_t *d_next = next_out, *d_label = label_out;
+  if (!out_dev) {
+    if (want_ring) {
+      if ((rc = ensure(W.out_next, N * 4))) return rc;
+      d_next = static_cast<uint32_t *>(W.out_next.ptr);
+    }
+    if (label_out) {
+      if ((rc = ensure(W.out_label, N * 4))) return rc;
+      d_label = static_cast<uint32_t *>(W.out_label.ptr);
+    }
+  }
+  if (!want_ring) d_next = nullptr;
+
+  // volume table: VolDev[nvol] then the varlink tables
+  {
+    std::vector<uint8_t> blob(nvol * sizeof(VolDev) + vt_entries * sizeof(double));
+    double *dtab = reinterpret_cast<double *>(static_cast<uint8_t *>(W.vols.ptr) + nvol * sizeof(VolDev));
+    size_t o = 0;
+    for (uint32_t v = 0; v < nvol; v++) {
+      VolDev V = hv[v].d;
+      if (vols[v].domain.varlink) {
+        V.vtab = dtab + o;
+        o += vols[v].domain.varlink->ntab;
+      }
+      memcpy(&blob[v * sizeof(VolDev)], &V, sizeof(VolDev));
+    }
+    if (vt_entries) memcpy(&blob[nvol * sizeof(VolDev)], vt_host.data(), vt_entries * sizeof(double));
+    FOF_HIP(hipMemcpyAsync(W.vols.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+    FOF_HIP(hipMemcpyAsync(W.vbeg.ptr, vbeg.data(), (nvol + 1) * 4, hipMemcpyHostToDevice, st));
+    FOF_HIP(hipStreamSynchronize(st));  // blob / vbeg are stack-owned
+  }
+
+  const VolDev *dV = static_cast<const VolDev *>(W.vols.ptr);
+  const uint32_t *dVB = static_cast<const uint32_t *>(W.vbeg.ptr);
+  float4 *pos4 = static_cast<float4 *>(W.pos4.ptr);
+  uint32_t *key = static_cast<uint32_t *>(W.key.ptr);
+  uint32_t *cnt = static_cast<uint32_t *>(W.cnt.ptr);
+  uint32_t *start = static_cast<uint32_t *>(W.start.ptr);
+  uint32_t *par = static_cast<uint32_t *>(W.par.ptr);
+  uint32_t *runend = static_cast<uint32_t *>(W.runend.ptr);
+  uint32_t *gsize = static_cast<uint32_t *>(W.gsize.ptr);
+  uint32_t *label = static_cast<uint32_t *>(W.label.ptr);
+  // four u32 arrays; viewed as float4[N] they are the sub-cell sort buffer
+  uint32_t *sk0 = static_cast<uint32_t *>(W.sortbuf.ptr);
+  uint32_t *sv0 = sk0 + q4, *sk1 = sk0 + 2 * q4, *sv1 = sk0 + 3 * q4;
+  uint32_t *dense = static_cast<uint32_t *>(W.dense.ptr);
+  uint32_t *small = static_cast<uint32_t *>(W.small.ptr);  // [0] err [1] ndense [2] M [3] H [4] overflow
+  unsigned long long *ctr = reinterpret_cast<unsigned long long *>(small + 16);  // 8 counters
+  const uint32_t NN = (uint32_t)N;
+  const uint32_t stride = P->stride, offpos = P->off_pos, offvel = P->off_vel;
+  const uint32_t gN = grid_for(N);
+
+  if (timing && !S.events) {
+    for (auto &e : S.ev) FOF_HIP(hipEventCreate(&e));
+    S.events = true;
+  }
+  auto mark = [&](int i) -> int {
+    if (timing) FOF_HIP(hipEventRecord(S.ev[i], st));
+    return 0;
+  };
+
+  FOF_HIP(hipMemsetAsync(small, 0, 256, st));
+  FOF_HIP(hipMemsetAsync(cnt, 0, C * 4, st));
+  if ((rc = mark(0))) return rc;
+  // 1-3: keys, scan, scatter
+  k_keys<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, key, cnt, T, dense,
+                             small + 1, small + 0);
+  FOF_HIP(hipGetLastError());
+  uint32_t hsmall[8];
+  FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
+  FOF_HIP(hipStreamSynchronize(st));
+  if (hsmall[0]) return FOF_E_DOMAIN;
+  const uint32_t ndense = hsmall[1];
+  if ((rc = scan_u32(cnt, start, C, W.tmp.ptr, st))) return rc;
+  k_scatter<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, key, start, cnt, pos4);
+  FOF_HIP(hipGetLastError());
+  k_init_uf<<<gN, 256, 0, st>>>(NN, par, runend);
+  FOF_HIP(hipGetLastError());
+  if ((rc = mark(1))) return rc;
+  // 4: sub-cell sort of dense cells (tmp4 aliases the sort buffers s0..s3)
+  const uint32_t gD = std::max(1u, std::min(ndense, 8192u));
+  if (ndense) {
+    k_subsort<<<gD, 256, 0, st>>>(dense, small + 1, start, dV, dVB, (int)nvol, pos4,
+                                  reinterpret_cast<float4 *>(sk0), par, runend);
+    FOF_HIP(hipGetLastError());
+  }
+  if ((rc = mark(2))) return rc;
+  // 5-6: linking
+  k_link_sparse<<<gN, 256, 0, st>>>(NN, pos4, start, runend, par, dV, dVB, (int)nvol, T, ctr);
+  FOF_HIP(hipGetLastError());
+  if (ndense) {
+    k_link_dense<<<gD, 256, 0, st>>>(dense, small + 1, pos4, start, runend, par, dV, dVB,
+                                     (int)nvol, ctr, small + 4);
+    FOF_HIP(hipGetLastError());
+  }
+  if ((rc = mark(3))) return rc;
+  // 7: union-find finish (minorig reuses runend, hsize reuses key)
+  uint32_t *minorig = runend, *hsize = key;
+  k_flatten<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, ctr);
+  FOF_HIP(hipGetLastError());
+  k_gather<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize);
+  FOF_HIP(hipGetLastError());
+  k_label<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, dV, dVB, (int)nvol, label, hsize);
+  FOF_HIP(hipGetLastError());
+  k_out_init<<<gN, 256, 0, st>>>(NN, label, dVB, (int)nvol, d_label, d_next);
+  FOF_HIP(hipGetLastError());
+  if ((rc = mark(4))) return rc;
+  // 8: kept groups sorted by label (stable: ascending index inside a group)
+  if ((rc = compact_flagged(label, N, kFlag, sk0, sv0, small + 2, W.tmp.ptr, st))) return rc;
+  FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
+  FOF_HIP(hipStreamSynchronize(st));
+  const uint32_t M = hsmall[2];
+  if (hsmall[4]) return FOF_E_RANGE;  // run table overflow (cannot happen with m <= 12)
+  if ((rc = radix_sort_u32(sk0, sv0, sk1, sv1, M, bits_for(N), W.tmp.ptr, st))) return rc;
+  if ((rc = mark(5))) return rc;
+  // 9: ring + halos
+  uint64_t H = 0;
+  if (M) {
+    k_ring<<<grid_for(M), 256, 0, st>>>(small + 2, sk0, sv0, hsize, dV, dVB, (int)nvol, d_next, sk1);
+    FOF_HIP(hipGetLastError());
+  }
+  if (want_halos && M) {
+    // halo starts: positions k of the heads of groups passing the size cut
+    if ((rc = compact_flagged(sk1, M, kFlag, nullptr, sv1, small + 3, W.tmp.ptr, st))) return rc;
+    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
+    FOF_HIP(hipStreamSynchronize(st));
+    H = hsmall[3];
+    if (nhalos_out) *nhalos_out = H;
+    if (halos && H > halo_cap) return FOF_E_RANGE;
+    if (H) {
+      fof_halo_v1 *hout = halos;
+      if (!out_dev || !halos) {
+        if ((rc = ensure(W.out_halo, H * sizeof(fof_halo_v1)))) return rc;
+        hout = static_cast<fof_halo_v1 *>(W.out_halo.ptr);
+      }
+      k_halo_sums<<<grid_for(H), 256, 0, st>>>(small + 3, sv1, sk0, sv0, hsize, dbase, stride,
+                                               offpos, offvel, dV, dVB, (int)nvol, hout,
+                                               static_cast<uint32_t *>(W.hhead.ptr), ctr);
+      FOF_HIP(hipGetLastError());
+      if (halos && !out_dev)
+        FOF_HIP(hipMemcpyAsync(halos, hout, H * sizeof(fof_halo_v1), hipMemcpyDeviceToHost, st));
+    }
+    if (halo_offsets) {
+      uint64_t *doff = static_cast<uint64_t *>(W.hoff.ptr);
+      k_halo_offsets<<<grid_for(nvol + 1), 256, 0, st>>>(
+          small + 3, static_cast<uint32_t *>(W.hhead.ptr), dVB, (int)nvol, doff);
+      FOF_HIP(hipGetLastError());
+      FOF_HIP(hipMemcpyAsync(halo_offsets, doff, (nvol + 1) * 8, hipMemcpyDeviceToHost, st));
+    }
+  } else {
+    if (nhalos_out) *nhalos_out = 0;
+    if (halo_offsets)
+      for (uint32_t v = 0; v <= nvol; v++) halo_offsets[v] = 0;
+  }
+  if ((rc = mark(6))) return rc;
+  if (!out_dev) {
+    if (want_ring) FOF_HIP(hipMemcpyAsync(next_out, d_next, N * 4, hipMemcpyDeviceToHost, st));
+    if (label_out) FOF_HIP(hipMemcpyAsync(label_out, d_label, N * 4, hipMemcpyDeviceToHost, st));
+  }
+  if ((rc = mark(7))) return rc;
+  unsigned long long hc[8];
+  FOF_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
+  FOF_HIP(hipStreamSynchronize(st));
+  if (stats) {
+    stats->ndense = ndense;
+    stats->pair_tests = hc[0];
+    stats->rechecks = hc[1];
+    stats->ngroups = hc[2];
+    stats->nmembers = M;
+    stats->nhalos = H;
+    stats->nhalos_owned = hc[3];
+    if (timing) {
+      float ms;
+      for (int i = 0; i < 7; i++) {
+        FOF_HIP(hipEventElapsedTime(&ms, S.ev[i], S.ev[i + 1]));
+        stats->phase_ms[i] = ms;
+      }
+      FOF_HIP(hipEventElapsedTime(&ms, S.ev[0], S.ev[7]));
+      stats->phase_ms[7] = ms;
+      FOF_HIP(hipEventElapsedTime(&ms, S.ev[2], S.ev[3]));
+      stats->link_kernel_ms = ms;
+    }
+  }
+  return 0;
+}
+
+extern "C" int fof_engine_init(fof_ctx *ctx) {
+  FOF_HIP(hipStreamCreate(&ctx->st.stream));
+  return 0;
+}
+
+extern "C" void fof_engine_free(fof_ctx *ctx) {
+  EngineState &S = ctx->st;
+  Scratch *all[] = {&S.w.pos4, &S.w.key, &S.w.cnt, &S.w.start, &S.w.par, &S.w.runend,
+                    &S.w.gsize, &S.w.label, &S.w.sortbuf,
+                    &S.w.dense, &S.w.tmp, &S.w.vols, &S.w.vbeg, &S.w.small, &S.w.input,
+                    &S.w.out_next, &S.w.out_label, &S.w.out_halo, &S.w.hhead, &S.w.hoff};
+  for (Scratch *s : all) release(*s);
+  if (S.events)
+    for (auto &e : S.ev) (void)hipEventDestroy(e);
+  S.events = false;
+  if (S.stream) (void)hipStreamDestroy(S.stream);
+  S.stream = nullptr;
+}

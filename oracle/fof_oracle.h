@@ -1,0 +1,98 @@
+/*
+ * fof_oracle.h -- CPU oracle for the CM_Halofinder Friends-of-Friends path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference's linking and catalogue arithmetic, used as the checker by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The
+ * product (cm_halofinder_amd) never links or calls it.
+ *
+ * Positions and velocities are passed as interleaved float xyz triples
+ * (pos[3*i+d]), i.e. the Pos/Vel members of the reference's particle record
+ * (src_v2/vars.h:36-47, MEMORY_MODE).  Indices are 0-based; the reference's
+ * 1-based next[i] is next0[i-1]+1.
+ *
+ * Pinning (what checks each function independently of this restatement):
+ *   or_pair_linked     exact ties d^2 == l^2 (Pythagorean quadruples at
+ *                      l = 5/16, 25/128), one-ulp offsets, NaN/duplicate rules
+ *   or_link_brute      O(n^2) over all pairs; no grid
+ *   or_link_uf         equals or_link_brute when l < lcell; invariants
+ *                      (permutation, lattice a<=l / a>l, chains)
+ *   or_link_ring       same partition as or_link_brute / or_link_uf
+ *   or_retrieve        closed forms (lattice clumps, symmetric groups),
+ *                      heads = minimum index, size cut, ownership boundaries
+ *   or_props_comp      compensated sums vs exact rational sums in tests
+ * The reference itself cannot be compiled here (vars.h includes GSL headers
+ * the image lacks), so no golden output of the reference exists.
+ */
+#ifndef CM_FOF_ORACLE_H
+#define CM_FOF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_domain {
+  double rmin[3], rmax[3], rmin_buff[3], rmax_buff[3];
+  double lcell[3];
+  double linklength;
+  double nphalomin;
+} or_domain;
+
+/* Variable linking length: l^2 of a cell = linear interpolation on
+ * uniform r2 samples of the squared distance of the cell centre to origin. */
+typedef struct or_varlink {
+  double origin[3];
+  double r2max;
+  uint32_t ntab;
+  const double *l2tab;
+} or_varlink;
+
+typedef struct or_halo {
+  uint32_t head;      /* 0-based minimum index                              */
+  uint32_t n;
+  uint32_t owned;
+  uint32_t edge_warn;
+  double sum_pos[3];  /* FP64, in ring order from the head                  */
+  double sum_vel[3];
+  float out[6];       /* Output_Halos values: (float)(sum * (1.0/n))        */
+} or_halo;
+
+/* Reference predicate, src_v2/FOF.c:351-357. */
+int or_pair_linked(const float *pi, const float *pj, double linksq);
+
+/* Squared linking length of reference cell (ix,iy,iz). */
+double or_cell_linksq(const or_domain *d, const or_varlink *vl, int ix, int iy,
+                      int iz);
+
+/* Linkers.  Return 0 or a negative error (-2: particle outside the buffered
+ * box or non-finite; -3: grid too large; -4: out of memory).  *pairs (may be
+ * NULL) receives the number of predicate evaluations. */
+int or_link_ring(uint32_t n, const float *pos, const or_domain *d,
+                 const or_varlink *vl, uint32_t *next0, uint64_t *pairs);
+int or_link_uf(uint32_t n, const float *pos, const or_domain *d,
+               const or_varlink *vl, uint32_t *label0, uint64_t *pairs);
+int or_link_brute(uint32_t n, const float *pos, const or_domain *d,
+                  const or_varlink *vl, uint32_t *label0);
+
+/* Ring <-> labels. */
+void or_labels_from_ring(uint32_t n, const uint32_t *next0, uint32_t *label0);
+void or_ring_from_labels(uint32_t n, const uint32_t *label0, uint32_t *next0);
+
+/* Retrieval + Checkhalo + Output_Halos sums over a ring.  Writes every group
+ * passing the size cut (owned or not) in ascending head order, up to cap;
+ * returns the number of such groups. */
+int64_t or_retrieve(uint32_t n, const uint32_t *next0, const float *pos,
+                    const float *vel, const or_domain *d, or_halo *out,
+                    uint64_t cap);
+
+/* Compensated (Neumaier) FP64 sums of pos/vel per group head: for each halo
+ * head h[k] (label value), sums over all i with label0[i]==h[k]. */
+void or_props_comp(uint32_t n, const uint32_t *label0, const float *pos,
+                   const float *vel, uint64_t nh, const uint32_t *heads,
+                   double *sum_pos, double *sum_vel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

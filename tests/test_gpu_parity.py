@@ -1,0 +1,209 @@
+"""GPU parity: the HIP linker against the CPU oracle through the C ABI.
+
+Bar: bit-exact labels (group minimum index), bit-exact successor ring
+(ascending index order inside each group) and bit-exact halo table (the
+oracle's retrieval / Checkhalo / Output_Halos arithmetic run over that same
+ring); halo sums within 1e-6 relative of compensated FP64 sums.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cm_halofinder_amd import _lib as L
+from cm_halofinder_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def odomain(v):
+    d = v.domain if hasattr(v, "domain") else v
+    return O.make_domain(d.rmin, d.rmax, d.rmin_buff, d.rmax_buff, d.lcell, d.linklength, d.nphalomin)
+
+
+def gpu_link(ctx, P, dom, flags=0, band=0.0, T=0):
+    n = len(P)
+    nxt = np.empty(n, np.uint32)
+    lab = np.empty(n, np.uint32)
+    halos = np.empty(max(n, 1), L.HALO_DTYPE)
+    st = L.Stats()
+    rc, nh = ctx.link(L.particles(P), dom, L.opts(flags, band, T), nxt, lab, halos, len(halos), st)
+    L.check(rc, "fof_link_gpu")
+    return lab, nxt, halos[:nh], st
+
+
+def check_against_oracle(ctx, P, dom, T=0, vl=None, ovl=None):
+    print(f"case n={len(P)} l={dom.linklength} T={T}", flush=True)
+    lab, nxt, halos, st = gpu_link(ctx, P, dom, T=T)
+    od = odomain(dom)
+    olab, pairs = O.link_uf(P["Pos"], od, ovl)
+    assert np.array_equal(lab, olab), f"labels differ at {np.flatnonzero(lab != olab)[:10]}"
+    oring = O.ring_from_labels(olab)
+    assert np.array_equal(nxt.astype(np.int64) - 1, oring.astype(np.int64))
+    oh = O.retrieve(oring, P["Pos"], P["Vel"], od)
+    assert len(oh) == len(halos)
+    for f in ("head", "n", "owned", "edge_warn"):
+        assert np.array_equal(halos[f], oh[f]), f
+    assert np.array_equal(halos["sum_pos"], oh["sum_pos"])
+    assert np.array_equal(halos["sum_vel"], oh["sum_vel"])
+    if len(oh):
+        sp, sv = O.props_compensated(olab, P["Pos"], P["Vel"], oh["head"])
+        rel = np.abs(halos["sum_pos"] - sp) / np.maximum(np.abs(sp), 1e-30)
+        assert rel.max() < 1e-6
+    return lab, st
+
+
+def test_c1_plain(ctx):
+    bx = W.plain(64)
+    _, P, vols = W.host_domain(bx)
+    lab, st = check_against_oracle(ctx, P, vols[0].domain)
+    assert st.ndense > 0 and st.pair_tests > 0
+    print("stats", st.as_dict())
+
+
+@pytest.mark.parametrize("T", [1, 4, 1 << 30])
+def test_dense_threshold_invariance(ctx, T):
+    bx = W.plain(24, seed=3)
+    _, P, vols = W.host_domain(bx)
+    check_against_oracle(ctx, P, vols[0].domain, T=T)
+
+
+@pytest.mark.parametrize("b", [0.05, 0.15, 0.3, 0.6, 1.2])
+def test_linking_lengths(ctx, b):
+    bx = W.plain(20, seed=4, b=b)
+    _, P, vols = W.host_domain(bx)
+    check_against_oracle(ctx, P, vols[0].domain)
+
+
+def test_random_dense_blobs(ctx):
+    rng = np.random.default_rng(9)
+    n = 20000
+    P = np.zeros(n, L.PART_DTYPE)
+    c = rng.random((30, 3)) * 20
+    pos = c[rng.integers(0, 30, n)] + rng.normal(0, 0.08, (n, 3))
+    P["Pos"] = np.clip(pos, 0, 20).astype(np.float32)
+    P["Vel"] = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    P["ID"] = np.arange(n)
+    dom = L.domain([0] * 3, [20] * 3, [0] * 3, [20] * 3, [1.0] * 3, 0.05, 5)
+    check_against_oracle(ctx, P, dom)
+
+
+def test_exact_ties(ctx):
+    # d^2 == l^2 exactly links (src_v2/FOF.c:357); one ulp further does not
+    l = 25.0 / 128.0
+    pts = []
+    base = np.array([512.0, 256.0, 300.0], np.float32)
+    for k in range(50):
+        o = base + np.float32(3.0 * k)
+        pts.append(o)
+        pts.append(o + np.array([15 / 128, 20 / 128, 0], np.float32))  # 3-4-5 triangle: d = l
+        far = o + np.array([0, 0, 1.5], np.float32)
+        pts.append(far)
+        q = far + np.array([l, 0, 0], np.float32)
+        q[0] = np.nextafter(q[0], np.float32(1e9))
+        pts.append(q)
+    P = np.zeros(len(pts), L.PART_DTYPE)
+    P["Pos"] = np.array(pts, np.float32)
+    dom = L.domain([500] * 3, [700] * 3, [500, 240, 290], [700, 450, 500], [0.9765625] * 3, l, 2)
+    lab, st = check_against_oracle(ctx, P, dom)
+    lab = lab.reshape(-1, 4)
+    assert np.all(lab[:, 1] == lab[:, 0]) and np.all(lab[:, 3] != lab[:, 2])
+    assert st.rechecks >= 100
+
+
+def test_precision_planted_pairs(ctx):
+    bx = W.precision(pside=64)
+    _, P, vols = W.host_domain(bx)
+    lab, st = check_against_oracle(ctx, P, vols[0].domain)
+    assert st.rechecks > 0.9 * bx.n * bx.pair_frac / 2
+
+
+def test_device_resident_matches_host(ctx):
+    import torch
+
+    bx = W.plain(32, seed=6)
+    _, P, vols = W.host_domain(bx)
+    dom = vols[0].domain
+    lab, nxt, halos, _ = gpu_link(ctx, P, dom)
+    dP = torch.from_numpy(P.view(np.uint8)).cuda()
+    n = len(P)
+    dnext = torch.empty(n, dtype=torch.int32, device="cuda")
+    dlab = torch.empty(n, dtype=torch.int32, device="cuda")
+    dh = torch.empty(n * 64, dtype=torch.uint8, device="cuda")
+    parts = L.particles(dP, n=n)
+    rc, nh = ctx.link(parts, dom, L.opts(L.OPT_OUT_DEVICE), dnext, dlab, dh, n)
+    L.check(rc)
+    torch.cuda.synchronize()
+    assert np.array_equal(dlab.cpu().numpy().view(np.uint32), lab)
+    assert np.array_equal(dnext.cpu().numpy().view(np.uint32), nxt)
+    hh = dh[: nh * 64].cpu().numpy().view(L.HALO_DTYPE)
+    assert np.array_equal(hh, halos)
+
+
+def test_batch_subvolumes(ctx):
+    bx = W.plain(40, seed=8)
+    recs = L.gen_host(bx.gen())
+    P, counts, vols = L.ghosts_host(bx.ghost(2), recs)
+    n = len(P)
+    nxt = np.empty(n, np.uint32)
+    lab = np.empty(n, np.uint32)
+    halos = np.empty(n, L.HALO_DTYPE)
+    hoff = np.zeros(len(vols) + 1, np.uint64)
+    st = L.Stats()
+    rc = ctx.link_batch(L.particles(P), vols, L.opts(), nxt, lab, halos, n, hoff, st)
+    L.check(rc)
+    for v, vol in enumerate(vols):
+        a, b = int(vol.offset), int(vol.offset + vol.n)
+        od = odomain(vol)
+        olab, _ = O.link_uf(P["Pos"][a:b], od)
+        assert np.array_equal(lab[a:b], olab), v
+        oring = O.ring_from_labels(olab)
+        assert np.array_equal(nxt[a:b].astype(np.int64) - 1, oring)
+        oh = O.retrieve(oring, P["Pos"][a:b], P["Vel"][a:b], od)
+        hv = halos[int(hoff[v]):int(hoff[v + 1])]
+        assert np.array_equal(hv["head"], oh["head"]) and np.array_equal(hv["sum_pos"], oh["sum_pos"])
+        assert np.array_equal(hv["owned"], oh["owned"])
+
+
+def test_domain_error(ctx):
+    P = np.zeros(10, L.PART_DTYPE)
+    P["Pos"] = 1.0
+    P["Pos"][3] = [5.0, 1.0, 1.0]
+    dom = L.domain([0] * 3, [2] * 3, [0] * 3, [2] * 3, [1.0] * 3, 0.1, 2)
+    rc, _ = ctx.link(L.particles(P), dom, None, None, None, None, 0)
+    assert rc == L.E_DOMAIN
+    P["Pos"][3] = [np.nan, 1.0, 1.0]
+    rc, _ = ctx.link(L.particles(P), dom, None, None, None, None, 0)
+    assert rc == L.E_DOMAIN
+    P["Pos"][3] = [1.5, 1.0, 1.0]
+    rc, _ = ctx.link(L.particles(P), dom, None, None, None, None, 0)
+    assert rc == 0
+
+
+def test_deterministic(ctx):
+    bx = W.plain(32, seed=12)
+    _, P, vols = W.host_domain(bx)
+    a = gpu_link(ctx, P, vols[0].domain)
+    b = gpu_link(ctx, P, vols[0].domain)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+
+
+def test_generator_and_ghosts_device_match_host():
+    import torch
+
+    bx = W.plain(32, seed=13)
+    g = bx.gen()
+    h = L.gen_host(g)
+    d = torch.empty(len(h) * 32, dtype=torch.uint8, device="cuda")
+    L.gen_device(g, d)
+    dh = d.cpu().numpy().view(L.PART_DTYPE)
+    assert np.array_equal(dh["ID"], h["ID"])
+    # device libm may round transcendental results differently in the last bit
+    assert np.abs(dh["Pos"] - h["Pos"]).max() <= 1e-4
+    hP, hc, hv = L.ghosts_host(bx.ghost(2), dh)
+    out = torch.empty(len(hP) * 32, dtype=torch.uint8, device="cuda")
+    dc, dv = L.ghosts_device(bx.ghost(2), d, len(h), out, len(hP))
+    assert np.array_equal(dc, hc)
+    assert np.array_equal(out.cpu().numpy().view(L.PART_DTYPE), hP)

@@ -468,7 +468,8 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
                                                     uint32_t *__restrict__ overflow) {
   __shared__ uint32_t ahead[kMaxRuns], aend[kMaxRuns], arel[kMaxRuns];
   __shared__ uint32_t bhead[kMaxRuns], bend[kMaxRuns], brel[kMaxRuns];
-  __shared__ uint32_t cnt;
+  __shared__ uint16_t aidx[kMaxRuns];
+  __shared__ uint32_t cnt, acnt;
   uint32_t tests = 0, rechecks = 0;
   const uint32_t nd = *ndense;
   for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
@@ -480,61 +481,65 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
     const int iy = (int)((lc / (uint32_t)V.nx) % (uint32_t)V.ny);
     const int iz = (int)(lc / ((uint32_t)V.nx * (uint32_t)V.ny));
     const Rule R = make_rule(V, ix, iy, iz);
-    const int mx = V.m[0], my = V.m[1], mz = V.m[2];
-    const int ox = ix * mx, oy = iy * my, oz = iz * mz;
-    // 14 home-cell pairs: (A,A) and the 13 forward cells.
+    const int mm[3] = {V.m[0], V.m[1], V.m[2]};
+    const int ox = ix * mm[0], oy = iy * mm[1], oz = iz * mm[2];
+    // the home cell's runs, once
+    const int na = collect_runs(pos4, run_end, sA, eA, V, ox, oy, oz, -1000, 1000, -1000, 1000, -1000,
+                                1000, ahead, aend, arel, &cnt);
+    if (na > kMaxRuns) {
+      if (threadIdx.x == 0) atomicOr(overflow, 1u);
+      continue;
+    }
+    int reach[3];
+    for (int a = 0; a < 3; a++)  // largest sub-cell offset whose gap can be within reach
+      reach[a] = min((int)floor(sqrt(R.prune2 / V.s2[a])) + 1, mm[a]);
+    // 14 home-cell pairs: (A,A) and the 13 forward cells of src_v2/FOF.c:122-153
     for (int q = 0; q < 14; q++) {
-      int dx = 0, dy = 0, dz = 0;
+      int dd[3] = {0, 0, 0};
       if (q > 0) {
-        // enumerate forward offsets in the order of src_v2/FOF.c:122-153
-        int k = q - 1;  // 0..12
-        if (k == 0) dx = 1;
-        else if (k < 4) dy = 1, dx = k - 2;
-        else {
-          dz = 1;
-          dy = (k - 4) / 3 - 1;
-          dx = (k - 4) % 3 - 1;
-        }
+        const int k = q - 1;  // 0..12
+        if (k == 0) dd[0] = 1;
+        else if (k < 4) dd[1] = 1, dd[0] = k - 2;
+        else dd[2] = 1, dd[1] = (k - 4) / 3 - 1, dd[0] = (k - 4) % 3 - 1;
       }
-      const int bx = ix + dx, by = iy + dy, bz = iz + dz;
+      const int bx = ix + dd[0], by = iy + dd[1], bz = iz + dd[2];
       if (bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz >= V.nz) continue;
       const uint32_t cb = cell_id(V, bx, by, bz);
       const uint32_t sB = start[cb], eB = start[cb + 1];
       if (sB == eB) continue;
-      // Face filter: only runs within two sub-cells of the shared boundary.
-      int alo[3] = {-1000, -1000, -1000}, ahi[3] = {1000, 1000, 1000};
-      int blo[3] = {-1000, -1000, -1000}, bhi[3] = {1000, 1000, 1000};
-      const int dd[3] = {dx, dy, dz}, mm[3] = {mx, my, mz};
-      for (int a = 0; a < 3; a++) {
-        // largest sub-cell offset whose gap can still be within reach
-        int reach = (int)floor(sqrt(R.prune2 / V.s2[a])) + 1;
-        reach = min(reach, mm[a]);
-        if (dd[a] == 1) {
-          alo[a] = mm[a] - reach;
-          blo[a] = mm[a];
-          bhi[a] = mm[a] - 1 + reach;
-        } else if (dd[a] == -1) {
-          ahi[a] = reach - 1;
-          blo[a] = -reach;
-          bhi[a] = -1;
+      int n1 = na, n2 = na;
+      const uint32_t *h2 = ahead, *e2 = aend, *r2 = arel;
+      if (q > 0) {
+        // only runs within reach of the shared face / edge / corner
+        int alo[3], ahi[3], blo[3], bhi[3];
+        for (int a = 0; a < 3; a++) {
+          alo[a] = -1000, ahi[a] = 1000, blo[a] = -1000, bhi[a] = 1000;
+          if (dd[a] == 1) alo[a] = mm[a] - reach[a], blo[a] = mm[a], bhi[a] = mm[a] - 1 + reach[a];
+          else if (dd[a] == -1) ahi[a] = reach[a] - 1, blo[a] = -reach[a], bhi[a] = -1;
         }
+        n2 = collect_runs(pos4, run_end, sB, eB, V, ox, oy, oz, blo[0], bhi[0], blo[1], bhi[1], blo[2],
+                          bhi[2], bhead, bend, brel, &cnt);
+        if (n2 > kMaxRuns) {
+          if (threadIdx.x == 0) atomicOr(overflow, 1u);
+          continue;
+        }
+        if (threadIdx.x == 0) acnt = 0;
+        __syncthreads();
+        for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
+          const uint32_t r = arel[ia];
+          const int rx = rel_x(r), ry = rel_y(r), rz = rel_z(r);
+          if (rx >= alo[0] && rx <= ahi[0] && ry >= alo[1] && ry <= ahi[1] && rz >= alo[2] &&
+              rz <= ahi[2])
+            aidx[atomicAdd(&acnt, 1u)] = (uint16_t)ia;
+        }
+        __syncthreads();
+        n1 = (int)acnt;
+        __syncthreads();
+        h2 = bhead, e2 = bend, r2 = brel;
       }
-      const int na = collect_runs(pos4, run_end, sA, eA, V, ox, oy, oz, alo[0], ahi[0], alo[1],
-                                  ahi[1], alo[2], ahi[2], ahead, aend, arel, &cnt);
-      int nb = 0;
-      if (q > 0)
-        nb = collect_runs(pos4, run_end, sB, eB, V, ox, oy, oz, blo[0], bhi[0], blo[1],
-                          bhi[1], blo[2], bhi[2], bhead, bend, brel, &cnt);
-      if (na > kMaxRuns || nb > kMaxRuns) {
-        if (threadIdx.x == 0) atomicOr(overflow, 1u);
-        continue;
-      }
-      if (na == 0 || (q > 0 && nb == 0)) continue;
-      const uint32_t *h2 = (q == 0) ? ahead : bhead;
-      const uint32_t *e2 = (q == 0) ? aend : bend;
-      const uint32_t *r2 = (q == 0) ? arel : brel;
-      const int n2 = (q == 0) ? na : nb;
-      for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
+      if (n1 == 0 || n2 == 0) continue;
+      for (int t = threadIdx.x; t < n1; t += blockDim.x) {
+        const int ia = (q == 0) ? t : (int)aidx[t];
         const uint32_t ra = arel[ia];
         const int ax = rel_x(ra), ay = rel_y(ra), az = rel_z(ra);
         const uint32_t ha = ahead[ia], ea = aend[ia];
@@ -565,6 +570,7 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
       }
       __syncthreads();
     }
+    __syncthreads();
   }
   add_counters(ctr, tests, rechecks);
 }

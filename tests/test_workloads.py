@@ -1,0 +1,78 @@
+"""Synthetic inputs and ghost shells (host implementations, no GPU)."""
+import numpy as np
+
+import oracle as O
+from cm_halofinder_amd import _lib as L
+from cm_halofinder_amd import workloads as W
+
+
+def test_generator_deterministic_and_shaped():
+    bx = W.plain(16, seed=3)
+    a = L.gen_host(bx.gen())
+    b = L.gen_host(bx.gen())
+    assert np.array_equal(a, b)
+    assert np.array_equal(np.sort(a["ID"]), np.arange(bx.n))  # IDs: a permutation
+    assert not np.array_equal(a["ID"], np.arange(bx.n))       # shuffled order
+    assert a["Pos"].min() >= 0 and a["Pos"].max() < bx.box
+    c = L.gen_host(W.plain(16, seed=4).gen())
+    assert not np.array_equal(a["Pos"], c["Pos"])
+    v = a["Vel"].reshape(-1)
+    assert 250 < v.std() < 350 and abs(v.mean()) < 20
+
+
+def test_clump_fraction_and_sizes():
+    bx = W.plain(32, seed=1)
+    recs = L.gen_host(bx.gen())
+    # half of the generation indices are clump members: they sit in compact clumps
+    ids = recs["ID"]
+    clumped = recs[ids < bx.n // 2]["Pos"]
+    bg = recs[ids >= bx.n // 2]["Pos"]
+    d = O.make_domain([0] * 3, [bx.box] * 3, [0] * 3, [bx.box] * 3, [bx.box / 32] * 3, bx.linklength, 20)
+    lab_c, _ = O.link_uf(clumped, d)
+    lab_b, _ = O.link_uf(bg, d)
+    sizes = np.bincount(lab_c)
+    assert (sizes >= 20).sum() > 0.8 * bx.nclumps  # most clumps are halos
+    assert np.bincount(lab_b).max() < 10           # the background does not percolate
+
+
+def test_borderline_pairs():
+    bx = W.precision(pside=16)
+    recs = L.gen_host(bx.gen())
+    g = bx.gen()
+    npairs = int(g.pair_frac * g.n / 2)
+    base = g.n - 2 * npairs
+    by_id = np.empty(g.n, np.int64)
+    by_id[recs["ID"]] = np.arange(g.n)
+    l2 = bx.linklength ** 2
+    rel = []
+    for q in range(npairs):
+        a, b = recs[by_id[base + 2 * q]]["Pos"], recs[by_id[base + 2 * q + 1]]["Pos"]
+        fx = (a - b).astype(np.float32).astype(np.float64)
+        d2 = (fx[0] * fx[0] + fx[1] * fx[1]) + fx[2] * fx[2]
+        rel.append(d2 / l2 - 1)
+    rel = np.abs(np.array(rel))
+    assert npairs > 10 and rel.max() < bx.pair_rel
+
+
+def test_ghost_shell_semantics():
+    bx = W.plain(16, seed=2)
+    recs = L.gen_host(bx.gen())
+    for g in (1, 2, 3):
+        out, counts, vols = L.ghosts_host(bx.ghost(g), recs)
+        assert counts.sum() == len(out)
+        owned_total = 0
+        for v in vols:
+            P = out[int(v.offset):int(v.offset + v.n)]
+            p = P["Pos"].astype(np.float64)
+            lo = np.array(v.domain.rmin_buff[:]), np.array(v.domain.rmax_buff[:])
+            assert np.all(p >= lo[0]) and np.all(p <= lo[1])
+            own = np.all((p >= np.array(v.domain.rmin[:])) & (p < np.array(v.domain.rmax[:])), 1)
+            owned_total += own.sum()
+            # every image is the owner's position shifted by 0 or +-box per axis
+            src = recs[np.argsort(recs["ID"])][P["ID"]]["Pos"].astype(np.float64)
+            s = (p - src) / bx.box
+            assert np.all(np.isin(np.round(s, 6), [-1.0, 0.0, 1.0]))
+            # a particle never appears twice at the same place in one volume
+            key = np.rec.fromarrays([P["ID"], P["Pos"][:, 0], P["Pos"][:, 1], P["Pos"][:, 2]])
+            assert len(np.unique(key)) == len(P)
+        assert owned_total == len(recs)  # each particle owned exactly once

@@ -191,6 +191,16 @@ def lib():
                 f"{LIB_PATH} is missing: build it with `python -m cm_halofinder_amd.build` "
                 "(there is no CPU fallback)"
             )
+        # One HIP runtime per process: torch's libtorch_hip NEEDs the
+        # unversioned "libamdhip64.so" and ships its own copy, while this
+        # library NEEDs "libamdhip64.so.7".  Loading torch first makes this
+        # library bind to torch's already-loaded runtime (same soname);
+        # loading it first would make torch load a second runtime, which then
+        # sees no devices.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, u32, u64 = C.c_void_p, C.c_uint32, C.c_uint64
         L.fof_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]

@@ -231,3 +231,62 @@ def test_generator_and_ghosts_device_match_host():
     dc, dv = L.ghosts_device(bx.ghost(2), d, len(h), out, len(hP))
     assert np.array_equal(dc, hc)
     assert np.array_equal(out.cpu().numpy().view(L.PART_DTYPE), hP)
+
+
+def lightcone_like(n, R, seed):
+    """Particles in a ball with density falling with radius, plus clumps."""
+    rng = np.random.default_rng(seed)
+    r = R * rng.random(n) ** (1 / 2.2)
+    v = rng.normal(size=(n, 3))
+    pos = v / np.linalg.norm(v, axis=1)[:, None] * r[:, None]
+    k = n // 3
+    cen = rng.normal(size=(60, 3))
+    cen = cen / np.linalg.norm(cen, axis=1)[:, None] * (R * rng.random(60))[:, None]
+    pos[:k] = cen[rng.integers(0, 60, k)] + rng.normal(0, 0.05 + 0.02 * np.abs(cen[rng.integers(0, 60, k)]).max(1)[:, None] / R, (k, 3))
+    P = np.zeros(n, L.PART_DTYPE)
+    P["Pos"] = np.clip(pos, -R, R).astype(np.float32)
+    P["Vel"] = rng.normal(0, 300, (n, 3)).astype(np.float32)
+    P["ID"] = np.arange(n)
+    return P
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_variable_linking_length(ctx, seed):
+    R = 20.0
+    P = lightcone_like(40000, R, seed)
+    lc = 2 * R / np.cbrt(len(P))
+    # l(r) = 0.2 n(r)^(-1/3) growing with radius, tabulated on r^2; the
+    # smallest l keeps the sub-cell path on (<= 12 sub-cells per axis)
+    r2max = 3 * R * R
+    r = np.sqrt(np.linspace(0, r2max, 257))
+    l = 0.2 * lc * (0.8 + (r / R) ** 0.8 * 0.45)
+    tab = l ** 2
+    vl = L.VarLinkTable([0.0, 0.0, 0.0], r2max, tab)
+    dom = L.domain([-R] * 3, [R] * 3, [-R] * 3, [R] * 3, [lc] * 3, 0.0, 5, vl)
+    ovl = O.varlink([0.0, 0.0, 0.0], r2max, tab)
+    od = odomain(dom)
+    lab, nxt, halos, st = gpu_link(ctx, P, dom)
+    olab, _ = O.link_uf(P["Pos"], od, ovl)
+    assert np.array_equal(lab, olab)
+    assert np.array_equal(O.link_brute(P["Pos"][:6000], od, ovl),
+                          O.link_uf(P["Pos"][:6000], od, ovl)[0])
+    assert st.ndense > 0 and len(np.unique(lab)) < len(lab) * 0.95
+
+
+def test_many_volumes(ctx):
+    P, vols = W.volumes_host(48, median=3000, lo=300, hi=30000, seed=3)
+    n = len(P)
+    nxt = np.empty(n, np.uint32)
+    lab = np.empty(n, np.uint32)
+    halos = np.empty(n, L.HALO_DTYPE)
+    hoff = np.zeros(len(vols) + 1, np.uint64)
+    st = L.Stats()
+    L.check(ctx.link_batch(L.particles(P), vols, L.opts(), nxt, lab, halos, n, hoff, st))
+    for v, vol in enumerate(vols):
+        a, b = int(vol.offset), int(vol.offset + vol.n)
+        od = odomain(vol)
+        olab, _ = O.link_uf(P["Pos"][a:b], od)
+        assert np.array_equal(lab[a:b], olab), v
+        oh = O.retrieve(O.ring_from_labels(olab), P["Pos"][a:b], P["Vel"][a:b], od)
+        hv = halos[int(hoff[v]):int(hoff[v + 1])]
+        assert np.array_equal(hv["head"], oh["head"]) and np.array_equal(hv["sum_vel"], oh["sum_vel"])

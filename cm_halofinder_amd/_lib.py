@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libcmfof.so")
 
 ABI_VERSION = 1
 MEM_HOST, MEM_DEVICE = 0, 1
-OPT_TIMING, OPT_OUT_DEVICE, OPT_NO_RING, OPT_NO_HALOS = 0x1, 0x2, 0x4, 0x8
+OPT_TIMING, OPT_OUT_DEVICE, OPT_NO_RING, OPT_NO_HALOS, OPT_SPARSE = 0x1, 0x2, 0x4, 0x8, 0x20
 
 E_ARG, E_DOMAIN, E_RANGE, E_NOMEM, E_HIP, E_NODEV = -1, -2, -3, -4, -5, -6
 

@@ -87,6 +87,78 @@ __device__ __forceinline__ uint32_t cell_id(const VolDev &V, int ix, int iy, int
          (uint32_t)ix;
 }
 
+// Cell addressing.  Dense: a table of cell starts over every cell of every
+// volume (cell id = cbeg + linear index).  Sparse: only occupied cells, in
+// (volume, linear index) order, found through an open-addressing hash table
+// keyed by the composite key (volume << volshift) | linear index.
+struct Grid {
+  const uint32_t *start;           // cell start offsets (+1 entry)
+  const unsigned long long *hkey;  // sparse: hash keys (empty = ~0)
+  const uint32_t *hval;            // sparse: unique cell index
+  const uint32_t *cell_of;         // sparse: cell of each sorted particle
+  unsigned long long hmask;
+  int sparse;
+  int volshift;
+};
+
+__device__ __forceinline__ unsigned long long mix64d(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ unsigned long long comp_key(const Grid &G, const VolDev &V, int ix,
+                                                       int iy, int iz) {
+  return ((unsigned long long)V.vid << G.volshift) |
+         (((unsigned long long)iz * (unsigned)V.ny + (unsigned)iy) * (unsigned)V.nx +
+          (unsigned)ix);
+}
+
+__device__ __forceinline__ uint32_t hlookup(const Grid &G, unsigned long long k) {
+  unsigned long long i = mix64d(k) & G.hmask;
+  for (;;) {
+    const unsigned long long kk = G.hkey[i];
+    if (kk == k) return G.hval[i];
+    if (kk == ~0ull) return kNone;
+    i = (i + 1) & G.hmask;
+  }
+}
+
+// Cell index of (ix,iy,iz) (in range), kNone when a sparse cell is empty.
+__device__ __forceinline__ uint32_t cell_index(const Grid &G, const VolDev &V, int ix, int iy,
+                                               int iz) {
+  if (!G.sparse) return cell_id(V, ix, iy, iz);
+  return hlookup(G, comp_key(G, V, ix, iy, iz));
+}
+
+// Particle range covering cells (x0..x1, iy, iz).  Sparse: the occupied
+// cells among consecutive keys are consecutive in the cell list.
+__device__ __forceinline__ void row_range(const Grid &G, const VolDev &V, int x0, int x1, int iy,
+                                          int iz, uint32_t &r0, uint32_t &r1) {
+  if (!G.sparse) {
+    const uint32_t c0 = cell_id(V, x0, iy, iz);
+    r0 = G.start[c0];
+    r1 = G.start[c0 + (uint32_t)(x1 - x0) + 1];
+    return;
+  }
+  uint32_t lo = kNone, hi = kNone;
+  const unsigned long long k0 = comp_key(G, V, x0, iy, iz);
+  for (int x = 0; x <= x1 - x0; x++) {
+    const uint32_t c = hlookup(G, k0 + (unsigned long long)x);
+    if (c != kNone) {
+      if (lo == kNone) lo = c;
+      hi = c;
+    }
+  }
+  if (lo == kNone) {
+    r0 = r1 = 0;
+    return;
+  }
+  r0 = G.start[lo];
+  r1 = G.start[hi + 1];
+}
+
 // Squared linking length of a home cell: constant, or the variable rule
 // (same operation order as the oracle; built with -ffp-contract=off).
 __device__ __forceinline__ double cell_linksq(const VolDev &V, int ix, int iy, int iz) {
@@ -242,6 +314,102 @@ __global__ __launch_bounds__(256) void k_scatter(const uint8_t *__restrict__ bas
   }
 }
 
+// 1-3 (sparse addressing): composite keys, sorted, occupied cells.
+__global__ __launch_bounds__(256) void k_keys_sparse(const uint8_t *__restrict__ base,
+                                                     uint32_t stride, uint32_t offpos, uint32_t N,
+                                                     const VolDev *__restrict__ vols,
+                                                     const uint32_t *__restrict__ vbeg, int nvol,
+                                                     Grid G, uint32_t *__restrict__ klo,
+                                                     uint32_t *__restrict__ khi,
+                                                     uint32_t *__restrict__ val,
+                                                     uint32_t *__restrict__ err) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += gridDim.x * blockDim.x) {
+    const VolDev &V = vols[vol_of(vbeg, nvol, p)];
+    const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
+    const float x = pp[0], y = pp[1], z = pp[2];
+    const bool ok = ((double)x >= V.rmin_buff[0]) && ((double)x <= V.rmax_buff[0]) &&
+                    ((double)y >= V.rmin_buff[1]) && ((double)y <= V.rmax_buff[1]) &&
+                    ((double)z >= V.rmin_buff[2]) && ((double)z <= V.rmax_buff[2]);
+    if (!ok) atomicOr(err, 1u);
+    double u;
+    const int ix = ok ? cell1((double)x, V.rmin_buff[0], V.lcell[0], V.nx, &u) : 0;
+    const int iy = ok ? cell1((double)y, V.rmin_buff[1], V.lcell[1], V.ny, &u) : 0;
+    const int iz = ok ? cell1((double)z, V.rmin_buff[2], V.lcell[2], V.nz, &u) : 0;
+    const unsigned long long k = comp_key(G, V, ix, iy, iz);
+    klo[p] = (uint32_t)k;
+    khi[p] = (uint32_t)(k >> 32);
+    val[p] = p;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather_u32(uint32_t N, const uint32_t *__restrict__ src,
+                                                    const uint32_t *__restrict__ idx,
+                                                    uint32_t *__restrict__ dst) {
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x)
+    dst[x] = src[idx[x]];
+}
+
+__global__ __launch_bounds__(256) void k_scatter_sparse(const uint8_t *__restrict__ base,
+                                                        uint32_t stride, uint32_t offpos,
+                                                        uint32_t N, const uint32_t *__restrict__ perm,
+                                                        const VolDev *__restrict__ vols,
+                                                        const uint32_t *__restrict__ vbeg,
+                                                        int nvol, Grid G,
+                                                        float4 *__restrict__ pos4,
+                                                        uint32_t *__restrict__ sklo,
+                                                        uint32_t *__restrict__ skhi) {
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += gridDim.x * blockDim.x) {
+    const uint32_t p = perm[x];
+    const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
+    const float4 q = make_float4(pp[0], pp[1], pp[2], __uint_as_float(p));
+    pos4[x] = q;
+    const VolDev &V = vols[vol_of(vbeg, nvol, p)];
+    const CellPos c = cell_pos(V, q.x, q.y, q.z);
+    const unsigned long long k = comp_key(G, V, c.ix, c.iy, c.iz);
+    sklo[x] = (uint32_t)k;
+    skhi[x] = (uint32_t)(k >> 32);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cell_flags(uint32_t N, const uint32_t *__restrict__ sklo,
+                                                    const uint32_t *__restrict__ skhi,
+                                                    uint32_t *__restrict__ flags) {
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += gridDim.x * blockDim.x)
+    flags[x] = (x == 0 || sklo[x] != sklo[x - 1] || skhi[x] != skhi[x - 1]) ? kFlag : 0u;
+}
+
+// Per occupied cell: close the start list, record each particle's cell,
+// insert the key into the hash table, list dense cells.
+__global__ __launch_bounds__(256) void k_cells_finish(
+    const uint32_t *__restrict__ dU, uint32_t N, uint32_t *__restrict__ cstart,
+    const uint32_t *__restrict__ sklo, const uint32_t *__restrict__ skhi, Grid G,
+    unsigned long long *__restrict__ hkey, uint32_t *__restrict__ hval,
+    uint32_t *__restrict__ cell_of, const VolDev *__restrict__ vols,
+    const uint32_t *__restrict__ vbeg, int nvol, uint32_t T, uint32_t *__restrict__ dense,
+    uint32_t *__restrict__ ndense) {
+  const uint32_t U = *dU;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < U;
+       c += gridDim.x * blockDim.x) {
+    const uint32_t s = cstart[c], e = (c + 1 < U) ? cstart[c + 1] : N;
+    if (c + 1 == U) cstart[U] = N;
+    for (uint32_t x = s; x < e; x++) cell_of[x] = c;
+    const unsigned long long k = (unsigned long long)sklo[s] | ((unsigned long long)skhi[s] << 32);
+    unsigned long long i = mix64d(k) & G.hmask;
+    for (;;) {
+      if (atomicCAS(hkey + i, ~0ull, k) == ~0ull) {
+        hval[i] = c;
+        break;
+      }
+      i = (i + 1) & G.hmask;
+    }
+    const VolDev &V = vols[vol_of(vbeg, nvol, s)];
+    if (V.m[0] > 0 && e - s >= T) dense[atomicAdd(ndense, 1u)] = c;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_init_uf(uint32_t N, uint32_t *__restrict__ par,
                                                  uint32_t *__restrict__ run_end) {
   for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
@@ -335,7 +503,7 @@ __global__ __launch_bounds__(256) void k_subsort(const uint32_t *__restrict__ de
 // 5: sparse cells -- one thread per particle.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *__restrict__ pos4,
-                                                     const uint32_t *__restrict__ start,
+                                                     const Grid G,
                                                      const uint32_t *__restrict__ run_end,
                                                      uint32_t *__restrict__ par,
                                                      const VolDev *__restrict__ vols,
@@ -348,31 +516,35 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
     const float4 px = pos4[x];
     const VolDev &V = vols[vol_of(vbeg, nvol, x)];
     const CellPos cp = cell_pos(V, px.x, px.y, px.z);
-    const uint32_t c = cell_id(V, cp.ix, cp.iy, cp.iz);
-    const uint32_t s = start[c], e = start[c + 1];
+    const uint32_t c = G.sparse ? G.cell_of[x] : cell_id(V, cp.ix, cp.iy, cp.iz);
+    const uint32_t s = G.start[c], e = G.start[c + 1];
     if (V.m[0] > 0 && e - s >= T) continue;  // dense: k_link_dense
     const Rule R = make_rule(V, cp.ix, cp.iy, cp.iz);
     const int nx = V.nx, ny = V.ny, nz = V.nz;
-    const uint32_t xl = (cp.ix > 0) ? 1u : 0u, xr = (cp.ix + 1 < nx) ? 1u : 0u;
+    const int xl = (cp.ix > 0) ? 1 : 0, xr = (cp.ix + 1 < nx) ? 1 : 0;
     // Up to five contiguous ranges cover the own cell (after x) and the 13
     // forward cells: (ix+1,iy,iz); row (iy+1,iz); rows (iy-1..iy+1, iz+1).
     uint32_t r0[5], r1[5];
     int nr = 0;
     r0[nr] = x + 1;
-    r1[nr] = start[c + 1 + xr];
+    if (!G.sparse) {
+      r1[nr] = G.start[c + 1 + xr];
+    } else {
+      r1[nr] = e;
+      if (xr) {
+        const uint32_t c1 = hlookup(G, comp_key(G, V, cp.ix + 1, cp.iy, cp.iz));
+        if (c1 != kNone) r1[nr] = G.start[c1 + 1];
+      }
+    }
     nr++;
     if (cp.iy + 1 < ny) {
-      const uint32_t cc = c + (uint32_t)nx;
-      r0[nr] = start[cc - xl];
-      r1[nr] = start[cc + xr + 1];
+      row_range(G, V, cp.ix - xl, cp.ix + xr, cp.iy + 1, cp.iz, r0[nr], r1[nr]);
       nr++;
     }
     if (cp.iz + 1 < nz) {
       for (int dy = -1; dy <= 1; dy++) {
         if (cp.iy + dy < 0 || cp.iy + dy >= ny) continue;
-        const uint32_t cc = c + (uint32_t)nx * (uint32_t)ny + (uint32_t)(dy * nx);
-        r0[nr] = start[cc - xl];
-        r1[nr] = start[cc + xr + 1];
+        row_range(G, V, cp.ix - xl, cp.ix + xr, cp.iy + dy, cp.iz + 1, r0[nr], r1[nr]);
         nr++;
       }
     }
@@ -458,7 +630,7 @@ __device__ int collect_runs(const float4 *__restrict__ pos4,
 __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__ dense,
                                                     const uint32_t *__restrict__ ndense,
                                                     const float4 *__restrict__ pos4,
-                                                    const uint32_t *__restrict__ start,
+                                                    const Grid G,
                                                     const uint32_t *__restrict__ run_end,
                                                     uint32_t *__restrict__ par,
                                                     const VolDev *__restrict__ vols,
@@ -474,12 +646,11 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
   const uint32_t nd = *ndense;
   for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
     const uint32_t c = dense[d];
-    const uint32_t sA = start[c], eA = start[c + 1];
+    const uint32_t sA = G.start[c], eA = G.start[c + 1];
     const VolDev &V = vols[vol_of(vbeg, nvol, sA)];
-    const uint32_t lc = c - V.cbeg;
-    const int ix = (int)(lc % (uint32_t)V.nx);
-    const int iy = (int)((lc / (uint32_t)V.nx) % (uint32_t)V.ny);
-    const int iz = (int)(lc / ((uint32_t)V.nx * (uint32_t)V.ny));
+    const float4 p0 = pos4[sA];
+    const CellPos c0 = cell_pos(V, p0.x, p0.y, p0.z);
+    const int ix = c0.ix, iy = c0.iy, iz = c0.iz;
     const Rule R = make_rule(V, ix, iy, iz);
     const int mm[3] = {V.m[0], V.m[1], V.m[2]};
     const int ox = ix * mm[0], oy = iy * mm[1], oz = iz * mm[2];
@@ -504,8 +675,9 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
       }
       const int bx = ix + dd[0], by = iy + dd[1], bz = iz + dd[2];
       if (bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz >= V.nz) continue;
-      const uint32_t cb = cell_id(V, bx, by, bz);
-      const uint32_t sB = start[cb], eB = start[cb + 1];
+      const uint32_t cb = cell_index(G, V, bx, by, bz);
+      if (cb == kNone) continue;
+      const uint32_t sB = G.start[cb], eB = G.start[cb + 1];
       if (sB == eB) continue;
       int n1 = na, n2 = na;
       const uint32_t *h2 = ahead, *e2 = aend, *r2 = arel;
@@ -802,6 +974,7 @@ static int bits_for(uint64_t n) {
 struct HostVol {
   VolDev d;
   uint64_t ncell;
+  double ncell_d;
 };
 
 int prepare_volume(const fof_volume_v1 &in, double band, HostVol &hv, std::vector<const fof_varlink_v1 *> &vl) {
@@ -827,9 +1000,10 @@ int prepare_volume(const fof_volume_v1 &in, double band, HostVol &hv, std::vecto
   // src_v2/FOF.c:63-65
   double e[3];
   for (int k = 0; k < 3; k++) e[k] = ceil((D.rmax_buff[k] - D.rmin_buff[k]) / D.lcell[k]) + 1.0;
-  if (e[0] * e[1] * e[2] > 2147483000.0) return FOF_E_RANGE;
+  if (!(e[0] < 2e9 && e[1] < 2e9 && e[2] < 2e9)) return FOF_E_RANGE;
   V.nx = (int)e[0], V.ny = (int)e[1], V.nz = (int)e[2];
-  hv.ncell = (uint64_t)V.nx * V.ny * V.nz;
+  hv.ncell_d = e[0] * e[1] * e[2];
+  hv.ncell = (hv.ncell_d < 1.8e19) ? (uint64_t)V.nx * (uint64_t)V.ny * (uint64_t)V.nz : ~0ull;
   // Smallest linking length that any home cell can use.
   double lmin2 = V.linksq;
   if (D.varlink) {
@@ -894,6 +1068,8 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
 
   // ---- volumes ----
   uint64_t N = 0, C = 0;
+  double Cd = 0;
+  int key_bits = 0;
   std::vector<HostVol> hv(nvol);
   std::vector<const fof_varlink_v1 *> vls;
   std::vector<uint32_t> vbeg(nvol + 1);
@@ -903,15 +1079,37 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     if (rc) return rc;
     vbeg[v] = (uint32_t)N;
     hv[v].d.pbeg = (uint32_t)N;
-    hv[v].d.cbeg = (uint32_t)C;
+    hv[v].d.vid = v;
     N += vols[v].n;
-    C += hv[v].ncell;
     hv[v].d.pend = (uint32_t)N;
-    hv[v].d.ncell = (uint32_t)hv[v].ncell;
-    if (N >= 0x7FFFFFFFull || C >= 0x7FFFFFF0ull) return FOF_E_RANGE;
+    Cd += hv[v].ncell_d;
+    if (N >= 0x7FFFFFFFull) return FOF_E_RANGE;
   }
   vbeg[nvol] = (uint32_t)N;
   if (N != P->n) return FOF_E_ARG;
+  // Dense cell tables unless they would dwarf the particle arrays.
+  const bool sparse = (flags & FOF_OPT_SPARSE) || Cd > 3.0 * (double)N + 4194304.0;
+  int volshift = 0;
+  if (sparse) {
+    int lb = 1, vb = 0;
+    for (uint32_t v = 0; v < nvol; v++) {
+      int b = 1;
+      while (b < 64 && (double)(1ull << b) < hv[v].ncell_d) b++;
+      lb = std::max(lb, b);
+    }
+    while ((1u << vb) < nvol) vb++;
+    if (lb + vb > 64 || hv[0].ncell_d > 1.8e19) return FOF_E_RANGE;
+    volshift = lb;
+    key_bits = lb + vb;
+  } else {
+    for (uint32_t v = 0; v < nvol; v++) {
+      hv[v].d.cbeg = (uint32_t)C;
+      hv[v].d.ncell = (uint32_t)hv[v].ncell;
+      C += hv[v].ncell;
+      if (C >= 0x7FFFFFF0ull) return FOF_E_RANGE;
+    }
+  }
+  const uint64_t Csz = sparse ? N : C;  // entries of the count and start tables
 
   // Variable linking-length tables are copied to the device back to back.
   size_t vt_entries = 0;
@@ -959,7 +1157,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     stats->version = FOF_ABI_VERSION;
     stats->nvol = nvol;
     stats->n = N;
-    stats->ncells = C;
+    stats->ncells = sparse ? 0 : C;
   }
   if (halo_offsets && !want_halos)
     for (uint32_t v = 0; v <= nvol; v++) halo_offsets[v] = 0;
@@ -975,8 +1173,8 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   int rc = 0;
   if ((rc = ensure(W.pos4, N * 16))) return rc;
   if ((rc = ensure(W.key, N * 4 + 64))) return rc;
-  if ((rc = ensure(W.cnt, C * 4 + 64))) return rc;
-  if ((rc = ensure(W.start, (C + 2) * 4 + 64))) return rc;
+  if ((rc = ensure(W.cnt, Csz * 4 + 64))) return rc;
+  if ((rc = ensure(W.start, (Csz + 2) * 4 + 64))) return rc;
   if ((rc = ensure(W.par, N * 4 + 64))) return rc;
   if ((rc = ensure(W.runend, N * 4 + 64))) return rc;
   if ((rc = ensure(W.gsize, N * 4 + 64))) return rc;
@@ -984,7 +1182,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   const uint64_t q4 = (N + 63) / 64 * 64;  // u32 entries per sort array
   if ((rc = ensure(W.sortbuf, 16 * q4 + 256))) return rc;
   if ((rc = ensure(W.dense, ndense_cap * 4 + 64))) return rc;
-  size_t tmpb = std::max(std::max(scan_tmp_bytes(C + 1), sort_tmp_bytes(N)), compact_tmp_bytes(N));
+  size_t tmpb = std::max(std::max(scan_tmp_bytes(Csz + 1), sort_tmp_bytes(N)), compact_tmp_bytes(N));
   if ((rc = ensure(W.tmp, tmpb))) return rc;
   if ((rc = ensure(W.vols, nvol * sizeof(VolDev) + vt_entries * sizeof(double) + 256))) return rc;
   if ((rc = ensure(W.vbeg, (nvol + 1) * 4 + 64))) return rc;
@@ -1061,20 +1259,70 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   };
 
   FOF_HIP(hipMemsetAsync(small, 0, 256, st));
-  FOF_HIP(hipMemsetAsync(cnt, 0, C * 4, st));
-  if ((rc = mark(0))) return rc;
-  // 1-3: keys, scan, scatter
-  k_keys<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, key, cnt, T, dense,
-                             small + 1, small + 0);
-  FOF_HIP(hipGetLastError());
+  Grid G;
+  G.start = start;
+  G.hkey = nullptr;
+  G.hval = nullptr;
+  G.cell_of = label;  // sparse: the label buffer is free until k_label
+  G.hmask = 0;
+  G.sparse = sparse ? 1 : 0;
+  G.volshift = volshift;
   uint32_t hsmall[8];
-  FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
-  FOF_HIP(hipStreamSynchronize(st));
-  if (hsmall[0]) return FOF_E_DOMAIN;
-  const uint32_t ndense = hsmall[1];
-  if ((rc = scan_u32(cnt, start, C, W.tmp.ptr, st))) return rc;
-  k_scatter<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, key, start, cnt, pos4);
-  FOF_HIP(hipGetLastError());
+  uint32_t ndense = 0;
+  if ((rc = mark(0))) return rc;
+  if (!sparse) {
+    // 1-3: keys + histogram, scan, counting-sort scatter
+    FOF_HIP(hipMemsetAsync(cnt, 0, C * 4, st));
+    k_keys<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, key, cnt, T,
+                               dense, small + 1, small + 0);
+    FOF_HIP(hipGetLastError());
+    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
+    FOF_HIP(hipStreamSynchronize(st));
+    if (hsmall[0]) return FOF_E_DOMAIN;
+    ndense = hsmall[1];
+    if ((rc = scan_u32(cnt, start, C, W.tmp.ptr, st))) return rc;
+    k_scatter<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, key, start, cnt, pos4);
+    FOF_HIP(hipGetLastError());
+  } else {
+    // 1-3: composite keys, LSD sort by (volume, cell), occupied cells, hash
+    k_keys_sparse<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, G, sk0, key,
+                                      sv0, small + 0);
+    FOF_HIP(hipGetLastError());
+    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
+    FOF_HIP(hipStreamSynchronize(st));
+    if (hsmall[0]) return FOF_E_DOMAIN;
+    if ((rc = radix_sort_u32(sk0, sv0, sk1, sv1, N, std::min(key_bits, 32), W.tmp.ptr, st))) return rc;
+    if (key_bits > 32) {
+      k_gather_u32<<<gN, 256, 0, st>>>(NN, key, sv0, gsize);
+      FOF_HIP(hipGetLastError());
+      if ((rc = radix_sort_u32(gsize, sv0, sk1, sv1, N, key_bits - 32, W.tmp.ptr, st))) return rc;
+    }
+    k_scatter_sparse<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, sv0, dV, dVB, (int)nvol, G, pos4,
+                                         par, runend);
+    FOF_HIP(hipGetLastError());
+    k_cell_flags<<<gN, 256, 0, st>>>(NN, par, runend, cnt);
+    FOF_HIP(hipGetLastError());
+    if ((rc = compact_flagged(cnt, N, kFlag, nullptr, start, small + 5, W.tmp.ptr, st))) return rc;
+    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
+    FOF_HIP(hipStreamSynchronize(st));
+    const uint32_t U = hsmall[5];
+    uint64_t hsz = 1024;
+    while (hsz < 2ull * U) hsz <<= 1;
+    if ((rc = ensure(W.hash, hsz * 12 + 256))) return rc;
+    unsigned long long *hk = static_cast<unsigned long long *>(W.hash.ptr);
+    uint32_t *hvv = reinterpret_cast<uint32_t *>(hk + hsz);
+    FOF_HIP(hipMemsetAsync(hk, 0xFF, hsz * 8, st));
+    G.hkey = hk;
+    G.hval = hvv;
+    G.hmask = hsz - 1;
+    k_cells_finish<<<grid_for(U), 256, 0, st>>>(small + 5, NN, start, par, runend, G, hk, hvv, label,
+                                                dV, dVB, (int)nvol, T, dense, small + 1);
+    FOF_HIP(hipGetLastError());
+    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
+    FOF_HIP(hipStreamSynchronize(st));
+    ndense = hsmall[1];
+    if (stats) stats->ncells = U;
+  }
   k_init_uf<<<gN, 256, 0, st>>>(NN, par, runend);
   FOF_HIP(hipGetLastError());
   if ((rc = mark(1))) return rc;
@@ -1087,10 +1335,10 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   }
   if ((rc = mark(2))) return rc;
   // 5-6: linking
-  k_link_sparse<<<gN, 256, 0, st>>>(NN, pos4, start, runend, par, dV, dVB, (int)nvol, T, ctr);
+  k_link_sparse<<<gN, 256, 0, st>>>(NN, pos4, G, runend, par, dV, dVB, (int)nvol, T, ctr);
   FOF_HIP(hipGetLastError());
   if (ndense) {
-    k_link_dense<<<gD, 256, 0, st>>>(dense, small + 1, pos4, start, runend, par, dV, dVB,
+    k_link_dense<<<gD, 256, 0, st>>>(dense, small + 1, pos4, G, runend, par, dV, dVB,
                                      (int)nvol, ctr, small + 4);
     FOF_HIP(hipGetLastError());
   }
@@ -1195,7 +1443,8 @@ extern "C" void fof_engine_free(fof_ctx *ctx) {
   Scratch *all[] = {&S.w.pos4, &S.w.key, &S.w.cnt, &S.w.start, &S.w.par, &S.w.runend,
                     &S.w.gsize, &S.w.label, &S.w.sortbuf,
                     &S.w.dense, &S.w.tmp, &S.w.vols, &S.w.vbeg, &S.w.small, &S.w.input,
-                    &S.w.out_next, &S.w.out_label, &S.w.out_halo, &S.w.hhead, &S.w.hoff};
+                    &S.w.out_next, &S.w.out_label, &S.w.out_halo, &S.w.hhead, &S.w.hoff,
+                    &S.w.hash};
   for (Scratch *s : all) release(*s);
   if (S.events)
     for (auto &e : S.ev) (void)hipEventDestroy(e);

@@ -38,7 +38,8 @@ struct alignas(16) VolDev {
   int m[3];            // sub-cells per axis; m[0] == 0 disables sub-cells
   uint32_t pbeg, pend; // particle range in the batch (input order and
                        // cell-sorted order coincide per volume)
-  uint32_t cbeg, ncell;
+  uint32_t cbeg, ncell; // dense cell addressing: first cell, cell count
+  uint32_t vid;         // volume index (sparse cell keys)
 };
 
 // Device-wide primitives (fof_prims.hip).
@@ -72,7 +73,7 @@ int compact_flagged(const uint32_t *src, uint64_t n, uint32_t flag, uint32_t *ou
 // Device workspace owned by a context (grow-only).
 struct Work {
   Scratch pos4, key, cnt, start, par, runend, gsize, label, sortbuf, dense, tmp, vols, vbeg,
-      small, input, out_next, out_label, out_halo, hhead, hoff;
+      small, input, out_next, out_label, out_halo, hhead, hoff, hash;
 };
 
 struct EngineState {

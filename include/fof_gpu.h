@@ -96,6 +96,10 @@ typedef struct fof_domain_v1 {
 #define FOF_OPT_NO_RING 0x4u    /* do not write next_out                      */
 #define FOF_OPT_NO_HALOS 0x8u   /* skip the halo table                        */
 #define FOF_OPT_SYNC_ONLY 0x10u /* reserved                                   */
+#define FOF_OPT_SPARSE 0x20u    /* address only occupied cells (sorted keys +
+                                   hash table); chosen automatically when a
+                                   dense cell table would be much larger than
+                                   the particle count                         */
 
 typedef struct fof_opts_v1 {
   uint32_t version;

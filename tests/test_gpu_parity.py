@@ -33,9 +33,9 @@ def gpu_link(ctx, P, dom, flags=0, band=0.0, T=0):
     return lab, nxt, halos[:nh], st
 
 
-def check_against_oracle(ctx, P, dom, T=0, vl=None, ovl=None):
-    print(f"case n={len(P)} l={dom.linklength} T={T}", flush=True)
-    lab, nxt, halos, st = gpu_link(ctx, P, dom, T=T)
+def check_against_oracle(ctx, P, dom, T=0, vl=None, ovl=None, flags=0):
+    print(f"case n={len(P)} l={dom.linklength} T={T} flags={flags}", flush=True)
+    lab, nxt, halos, st = gpu_link(ctx, P, dom, flags=flags, T=T)
     od = odomain(dom)
     olab, pairs = O.link_uf(P["Pos"], od, ovl)
     assert np.array_equal(lab, olab), f"labels differ at {np.flatnonzero(lab != olab)[:10]}"
@@ -290,3 +290,45 @@ def test_many_volumes(ctx):
         oh = O.retrieve(O.ring_from_labels(olab), P["Pos"][a:b], P["Vel"][a:b], od)
         hv = halos[int(hoff[v]):int(hoff[v + 1])]
         assert np.array_equal(hv["head"], oh["head"]) and np.array_equal(hv["sum_vel"], oh["sum_vel"])
+
+
+@pytest.mark.parametrize("scale,b", [(32, 0.2), (20, 0.6), (24, 0.1)])
+def test_sparse_cells(ctx, scale, b):
+    bx = W.plain(scale, seed=15, b=b)
+    _, P, vols = W.host_domain(bx)
+    lab, st = check_against_oracle(ctx, P, vols[0].domain, flags=L.OPT_SPARSE)
+    assert st.ncells > 0
+
+
+def test_sparse_variable_linking_length(ctx):
+    R = 20.0
+    P = lightcone_like(30000, R, 5)
+    lc = 2 * R / np.cbrt(len(P))
+    r2max = 3 * R * R
+    r = np.sqrt(np.linspace(0, r2max, 129))
+    tab = (0.2 * lc * (0.8 + (r / R) ** 0.8 * 0.45)) ** 2
+    vl = L.VarLinkTable([0.0, 0.0, 0.0], r2max, tab)
+    dom = L.domain([-R] * 3, [R] * 3, [-R] * 3, [R] * 3, [lc] * 3, 0.0, 5, vl)
+    lab, _, _, st = gpu_link(ctx, P, dom, flags=L.OPT_SPARSE)
+    olab, _ = O.link_uf(P["Pos"], odomain(dom), O.varlink([0.0, 0.0, 0.0], r2max, tab))
+    assert np.array_equal(lab, olab)
+    assert st.ndense > 0
+
+
+def test_sparse_batch(ctx):
+    P, vols = W.volumes_host(24, median=2000, lo=300, hi=20000, seed=9)
+    n = len(P)
+    outs = []
+    for fl in (0, L.OPT_SPARSE):
+        lab = np.empty(n, np.uint32)
+        nxt = np.empty(n, np.uint32)
+        halos = np.empty(n, L.HALO_DTYPE)
+        hoff = np.zeros(len(vols) + 1, np.uint64)
+        L.check(ctx.link_batch(L.particles(P), vols, L.opts(fl), nxt, lab, halos, n, hoff))
+        outs.append((lab, nxt, halos[: int(hoff[-1])], hoff))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+    for v, vol in enumerate(vols[:6]):
+        a, b = int(vol.offset), int(vol.offset + vol.n)
+        olab, _ = O.link_uf(P["Pos"][a:b], odomain(vol))
+        assert np.array_equal(outs[1][0][a:b], olab)

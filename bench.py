@@ -200,10 +200,10 @@ def main():
         L.check(rc, "batched")
 
     t_plain = run_steps(step_plain, args.warmup, args.steps)
-    log(f"plain: {t_plain * 1e3:.1f} ms/step  phases {[round(x, 2) for x in st1.phase_ms]}")
+    log(f"plain: {t_plain * 1e3:.1f} ms/step  phases {dict(zip(L.PHASES, [round(x, 2) for x in st1.phase_ms]))}")
     fp1 = group_fingerprints(D1, N1, lab, hal, nh1[0])
     t_batch = run_steps(step_batch, args.warmup, args.steps)
-    log(f"batched: {t_batch * 1e3:.1f} ms/step  phases {[round(x, 2) for x in stB.phase_ms]}")
+    log(f"batched: {t_batch * 1e3:.1f} ms/step  phases {dict(zip(L.PHASES, [round(x, 2) for x in stB.phase_ms]))}")
     nhB = int(hoff[-1])
     fpB = group_fingerprints(DB, NB, lab, hal, nhB)
     same_groups = bool(fp1.shape == fpB.shape and torch.equal(fp1, fpB))
@@ -239,8 +239,10 @@ def main():
                         f"batched = {args.g}^3 subvolumes",
             "particles_with_shells": {"plain": N1, "batched": NB},
         },
-        "plain": {"value": n / t_plain, "ms_per_step": t_plain * 1e3, "phase_ms": list(st1.phase_ms)},
-        "batched": {"value": value, "ms_per_step": t_batch * 1e3, "phase_ms": list(stB.phase_ms)},
+        "plain": {"value": n / t_plain, "ms_per_step": t_plain * 1e3,
+                  "phase_ms": {k: round(v, 3) for k, v in zip(L.PHASES, st1.phase_ms)}},
+        "batched": {"value": value, "ms_per_step": t_batch * 1e3,
+                    "phase_ms": {k: round(v, 3) for k, v in zip(L.PHASES, stB.phase_ms)}},
         "speedup_batched_vs_plain": t_plain / t_batch,
         "groups_identical_plain_vs_batched": same_groups,
         "stats": {"pair_tests": pair_tests, "rechecks": stB.rechecks, "ndense": stB.ndense,

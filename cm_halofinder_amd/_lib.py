@@ -105,7 +105,9 @@ HALO_DTYPE = np.dtype(
 )
 assert HALO_DTYPE.itemsize == C.sizeof(Halo) == 64
 
-NPHASE = 8
+NPHASE = 16
+PHASES = ("keys", "scan", "scatter", "uf_init", "subsort", "link_sparse", "link_dense",
+          "flatten", "label", "compact", "ring_sort", "ring", "halo_list", "halo_sums", "output", "total")
 
 
 class Stats(C.Structure):
@@ -121,13 +123,15 @@ class Stats(C.Structure):
         ("nmembers", C.c_uint64),
         ("nhalos", C.c_uint64),
         ("nhalos_owned", C.c_uint64),
+        ("sparse", C.c_uint32),
+        ("reserved", C.c_uint32),
         ("phase_ms", C.c_double * NPHASE),
         ("link_kernel_ms", C.c_double),
     ]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "phase_ms"}
-        d["phase_ms"] = list(self.phase_ms)
+        d["phase_ms"] = dict(zip(PHASES, self.phase_ms))
         return d
 
 

@@ -1253,8 +1253,22 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     for (auto &e : S.ev) FOF_HIP(hipEventCreate(&e));
     S.events = true;
   }
-  auto mark = [&](int i) -> int {
-    if (timing) FOF_HIP(hipEventRecord(S.ev[i], st));
+  // Timing spans: events on the engine stream around each kernel group.
+  uint32_t spans = 0;
+  int first_span = -1, last_span = -1;
+  auto tb = [&](int k) -> int {
+    if (timing) {
+      FOF_HIP(hipEventRecord(S.ev[2 * k], st));
+      if (first_span < 0) first_span = k;
+    }
+    return 0;
+  };
+  auto te = [&](int k) -> int {
+    if (timing) {
+      FOF_HIP(hipEventRecord(S.ev[2 * k + 1], st));
+      spans |= 1u << k;
+      last_span = k;
+    }
     return 0;
   };
 
@@ -1269,37 +1283,48 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   G.volshift = volshift;
   uint32_t hsmall[8];
   uint32_t ndense = 0;
-  if ((rc = mark(0))) return rc;
   if (!sparse) {
     // 1-3: keys + histogram, scan, counting-sort scatter
     FOF_HIP(hipMemsetAsync(cnt, 0, C * 4, st));
+    if ((rc = tb(0))) return rc;
     k_keys<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, key, cnt, T,
                                dense, small + 1, small + 0);
     FOF_HIP(hipGetLastError());
+    if ((rc = te(0))) return rc;
     FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
     FOF_HIP(hipStreamSynchronize(st));
     if (hsmall[0]) return FOF_E_DOMAIN;
     ndense = hsmall[1];
+    if ((rc = tb(1))) return rc;
     if ((rc = scan_u32(cnt, start, C, W.tmp.ptr, st))) return rc;
+    if ((rc = te(1))) return rc;
+    if ((rc = tb(2))) return rc;
     k_scatter<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, key, start, cnt, pos4);
     FOF_HIP(hipGetLastError());
+    if ((rc = te(2))) return rc;
   } else {
     // 1-3: composite keys, LSD sort by (volume, cell), occupied cells, hash
+    if ((rc = tb(0))) return rc;
     k_keys_sparse<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, G, sk0, key,
                                       sv0, small + 0);
     FOF_HIP(hipGetLastError());
+    if ((rc = te(0))) return rc;
     FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
     FOF_HIP(hipStreamSynchronize(st));
     if (hsmall[0]) return FOF_E_DOMAIN;
+    if ((rc = tb(1))) return rc;
     if ((rc = radix_sort_u32(sk0, sv0, sk1, sv1, N, std::min(key_bits, 32), W.tmp.ptr, st))) return rc;
     if (key_bits > 32) {
       k_gather_u32<<<gN, 256, 0, st>>>(NN, key, sv0, gsize);
       FOF_HIP(hipGetLastError());
       if ((rc = radix_sort_u32(gsize, sv0, sk1, sv1, N, key_bits - 32, W.tmp.ptr, st))) return rc;
     }
+    if ((rc = te(1))) return rc;
+    if ((rc = tb(2))) return rc;
     k_scatter_sparse<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, sv0, dV, dVB, (int)nvol, G, pos4,
                                          par, runend);
     FOF_HIP(hipGetLastError());
+    if ((rc = te(2))) return rc;
     k_cell_flags<<<gN, 256, 0, st>>>(NN, par, runend, cnt);
     FOF_HIP(hipGetLastError());
     if ((rc = compact_flagged(cnt, N, kFlag, nullptr, start, small + 5, W.tmp.ptr, st))) return rc;
@@ -1323,54 +1348,69 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     ndense = hsmall[1];
     if (stats) stats->ncells = U;
   }
+  if ((rc = tb(3))) return rc;
   k_init_uf<<<gN, 256, 0, st>>>(NN, par, runend);
   FOF_HIP(hipGetLastError());
-  if ((rc = mark(1))) return rc;
+  if ((rc = te(3))) return rc;
   // 4: sub-cell sort of dense cells (tmp4 aliases the sort buffers s0..s3)
   const uint32_t gD = std::max(1u, std::min(ndense, 8192u));
   if (ndense) {
+    if ((rc = tb(4))) return rc;
     k_subsort<<<gD, 256, 0, st>>>(dense, small + 1, start, dV, dVB, (int)nvol, pos4,
                                   reinterpret_cast<float4 *>(sk0), par, runend);
     FOF_HIP(hipGetLastError());
+    if ((rc = te(4))) return rc;
   }
-  if ((rc = mark(2))) return rc;
   // 5-6: linking
+  if ((rc = tb(5))) return rc;
   k_link_sparse<<<gN, 256, 0, st>>>(NN, pos4, G, runend, par, dV, dVB, (int)nvol, T, ctr);
   FOF_HIP(hipGetLastError());
+  if ((rc = te(5))) return rc;
   if (ndense) {
+    if ((rc = tb(6))) return rc;
     k_link_dense<<<gD, 256, 0, st>>>(dense, small + 1, pos4, G, runend, par, dV, dVB,
                                      (int)nvol, ctr, small + 4);
     FOF_HIP(hipGetLastError());
+    if ((rc = te(6))) return rc;
   }
-  if ((rc = mark(3))) return rc;
   // 7: union-find finish (minorig reuses runend, hsize reuses key)
   uint32_t *minorig = runend, *hsize = key;
+  if ((rc = tb(7))) return rc;
   k_flatten<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, ctr);
   FOF_HIP(hipGetLastError());
   k_gather<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize);
   FOF_HIP(hipGetLastError());
+  if ((rc = te(7))) return rc;
+  if ((rc = tb(8))) return rc;
   k_label<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, dV, dVB, (int)nvol, label, hsize);
   FOF_HIP(hipGetLastError());
   k_out_init<<<gN, 256, 0, st>>>(NN, label, dVB, (int)nvol, d_label, d_next);
   FOF_HIP(hipGetLastError());
-  if ((rc = mark(4))) return rc;
+  if ((rc = te(8))) return rc;
   // 8: kept groups sorted by label (stable: ascending index inside a group)
+  if ((rc = tb(9))) return rc;
   if ((rc = compact_flagged(label, N, kFlag, sk0, sv0, small + 2, W.tmp.ptr, st))) return rc;
+  if ((rc = te(9))) return rc;
   FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
   FOF_HIP(hipStreamSynchronize(st));
   const uint32_t M = hsmall[2];
   if (hsmall[4]) return FOF_E_RANGE;  // run table overflow (cannot happen with m <= 12)
+  if ((rc = tb(10))) return rc;
   if ((rc = radix_sort_u32(sk0, sv0, sk1, sv1, M, bits_for(N), W.tmp.ptr, st))) return rc;
-  if ((rc = mark(5))) return rc;
+  if ((rc = te(10))) return rc;
   // 9: ring + halos
   uint64_t H = 0;
   if (M) {
+    if ((rc = tb(11))) return rc;
     k_ring<<<grid_for(M), 256, 0, st>>>(small + 2, sk0, sv0, hsize, dV, dVB, (int)nvol, d_next, sk1);
     FOF_HIP(hipGetLastError());
+    if ((rc = te(11))) return rc;
   }
   if (want_halos && M) {
     // halo starts: positions k of the heads of groups passing the size cut
+    if ((rc = tb(12))) return rc;
     if ((rc = compact_flagged(sk1, M, kFlag, nullptr, sv1, small + 3, W.tmp.ptr, st))) return rc;
+    if ((rc = te(12))) return rc;
     FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
     FOF_HIP(hipStreamSynchronize(st));
     H = hsmall[3];
@@ -1382,10 +1422,12 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
         if ((rc = ensure(W.out_halo, H * sizeof(fof_halo_v1)))) return rc;
         hout = static_cast<fof_halo_v1 *>(W.out_halo.ptr);
       }
+      if ((rc = tb(13))) return rc;
       k_halo_sums<<<grid_for(H), 256, 0, st>>>(small + 3, sv1, sk0, sv0, hsize, dbase, stride,
                                                offpos, offvel, dV, dVB, (int)nvol, hout,
                                                static_cast<uint32_t *>(W.hhead.ptr), ctr);
       FOF_HIP(hipGetLastError());
+      if ((rc = te(13))) return rc;
       if (halos && !out_dev)
         FOF_HIP(hipMemcpyAsync(halos, hout, H * sizeof(fof_halo_v1), hipMemcpyDeviceToHost, st));
     }
@@ -1401,12 +1443,12 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     if (halo_offsets)
       for (uint32_t v = 0; v <= nvol; v++) halo_offsets[v] = 0;
   }
-  if ((rc = mark(6))) return rc;
+  if ((rc = tb(14))) return rc;
   if (!out_dev) {
     if (want_ring) FOF_HIP(hipMemcpyAsync(next_out, d_next, N * 4, hipMemcpyDeviceToHost, st));
     if (label_out) FOF_HIP(hipMemcpyAsync(label_out, d_label, N * 4, hipMemcpyDeviceToHost, st));
   }
-  if ((rc = mark(7))) return rc;
+  if ((rc = te(14))) return rc;
   unsigned long long hc[8];
   FOF_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
   FOF_HIP(hipStreamSynchronize(st));
@@ -1418,16 +1460,19 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     stats->nmembers = M;
     stats->nhalos = H;
     stats->nhalos_owned = hc[3];
+    stats->sparse = sparse ? 1u : 0u;
     if (timing) {
       float ms;
-      for (int i = 0; i < 7; i++) {
-        FOF_HIP(hipEventElapsedTime(&ms, S.ev[i], S.ev[i + 1]));
-        stats->phase_ms[i] = ms;
+      for (int k = 0; k < FOF_NPHASE - 1; k++) {
+        if (!(spans & (1u << k))) continue;
+        FOF_HIP(hipEventElapsedTime(&ms, S.ev[2 * k], S.ev[2 * k + 1]));
+        stats->phase_ms[k] = ms;
       }
-      FOF_HIP(hipEventElapsedTime(&ms, S.ev[0], S.ev[7]));
-      stats->phase_ms[7] = ms;
-      FOF_HIP(hipEventElapsedTime(&ms, S.ev[2], S.ev[3]));
-      stats->link_kernel_ms = ms;
+      if (first_span >= 0) {
+        FOF_HIP(hipEventElapsedTime(&ms, S.ev[2 * first_span], S.ev[2 * last_span + 1]));
+        stats->phase_ms[FOF_NPHASE - 1] = ms;
+      }
+      stats->link_kernel_ms = stats->phase_ms[5] + stats->phase_ms[6];
     }
   }
   return 0;

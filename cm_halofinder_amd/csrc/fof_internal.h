@@ -79,7 +79,7 @@ struct Work {
 struct EngineState {
   Work w;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[FOF_NPHASE + 2] = {};
+  hipEvent_t ev[2 * FOF_NPHASE] = {};
   bool events = false;
 };
 

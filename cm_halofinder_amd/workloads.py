@@ -121,3 +121,138 @@ def volumes_host(nvol: int, median: float, lo: float, hi: float, seed: int = 5):
         parts.append(out)
         vols.append(v)
     return np.concatenate(parts), vols
+
+
+# ---------------------------------------------------------------------------
+# Lightcone: full-sky ball, radial shells, variable linking length
+# ---------------------------------------------------------------------------
+C_OVER_H0 = 2997.92458  # Mpc/h
+
+
+def comoving_distance_table(om: float = 0.31, zmax: float = 6.0, nz: int = 60001):
+    """(z, r(z)) for flat LCDM: r = c/H0 * int_0^z dz / sqrt(om (1+z)^3 + 1 - om)."""
+    z = np.linspace(0.0, zmax, nz)
+    ez = 1.0 / np.sqrt(om * (1 + z) ** 3 + 1 - om)
+    r = C_OVER_H0 * np.concatenate([[0.0], np.cumsum(0.5 * (ez[1:] + ez[:-1]) * np.diff(z))])
+    return z, r
+
+
+@dataclasses.dataclass
+class Lightcone:
+    n: int = 400_000_000
+    rmax: float = 3000.0
+    nshell: int = 64
+    om: float = 0.31
+    b: float = 0.2
+    seed: int = 17
+    nphalomin: float = 20.0
+    ntab: int = 4096
+
+    def __post_init__(self):
+        z, r = comoving_distance_table(self.om)
+        self._z, self._r = z, r
+        rr = np.linspace(0.0, self.rmax, 200001)
+        zz = np.interp(rr, r, z)
+        w = rr ** 2 * (1 + zz) ** -1.5
+        cdf = np.concatenate([[0.0], np.cumsum(0.5 * (w[1:] + w[:-1]) * np.diff(rr))])
+        self.n0 = self.n / (4 * np.pi * cdf[-1])  # n(r) = n0 (1+z)^-1.5
+        self._rr, self._cdf = rr, cdf / cdf[-1]
+
+    def z_of_r(self, r):
+        return np.interp(r, self._r, self._z)
+
+    def density(self, r):
+        return self.n0 * (1 + self.z_of_r(r)) ** -1.5
+
+    def link(self, r):
+        """l(r) = b n(r)^(-1/3)."""
+        return self.b * self.density(r) ** (-1.0 / 3.0)
+
+    def sample_r(self, u):
+        return np.interp(u, self._cdf, self._rr)
+
+    @property
+    def ghost(self):
+        # wider than any clump: radius <= 12.2 a with a <= 0.12 l (5000*2/100)^(1/3) < 0.6 l
+        return 8.0 * float(self.link(self.rmax))
+
+    def varlink_table(self):
+        r2max = (self.rmax + self.ghost) ** 2 * 3.0
+        r = np.sqrt(np.linspace(0.0, r2max, self.ntab))
+        return r2max, self.link(np.minimum(r, self.rmax * 1.5)) ** 2
+
+    def shells(self):
+        """Shell edges at equal expected particle counts."""
+        return self.sample_r(np.linspace(0.0, 1.0, self.nshell + 1))
+
+
+def lightcone_particles(lc: Lightcone) -> np.ndarray:
+    """Host generator (numpy): half background following n(r), half in
+    Plummer clumps (same mass function) whose centres follow n(r), so the
+    clump abundance tracks n(r); scale radius a = 0.12 l(r) (size/100)^(1/3),
+    i.e. compact relative to the local linking length at every redshift."""
+    rng = np.random.default_rng(lc.seed)
+    n = lc.n
+    nbg = n // 2
+    out = np.zeros(n, L.PART_DTYPE)
+
+    def dirs(k):
+        v = rng.normal(size=(k, 3))
+        return v / np.linalg.norm(v, axis=1)[:, None]
+
+    rb = lc.sample_r(rng.random(nbg))
+    pos = dirs(nbg) * rb[:, None]
+    # clumps: quantile sizes of M^-1.9 on [20, 5000] rescaled to the clumped count
+    ncl = n - nbg
+    K = max(1, int(round(n * CLUMPS_PER_PARTICLE)))
+    u = (np.arange(K) + 0.5) / K
+    a1 = 1 - 1.9
+    m = (20.0 ** a1 + u * (5000.0 ** a1 - 20.0 ** a1)) ** (1 / a1)
+    sz = np.maximum(1, np.floor(m * ncl / m.sum())).astype(np.int64)
+    sz[: ncl - sz.sum()] += 1 if ncl > sz.sum() else 0
+    while sz.sum() > ncl:
+        sz[np.argmax(sz)] -= 1
+    rc = lc.sample_r(rng.random(K))
+    cen = dirs(K) * rc[:, None]
+    a = 0.12 * lc.link(rc) * (sz / 100.0) ** (1 / 3)
+    who = np.repeat(np.arange(K), sz)
+    uu = 0.99 * rng.random(len(who))
+    rr = a[who] / np.sqrt(uu ** (-2 / 3) - 1)
+    pc = cen[who] + dirs(len(who)) * rr[:, None]
+    allp = np.concatenate([pos, pc])
+    rad = np.linalg.norm(allp, axis=1)
+    keep = rad < lc.rmax
+    allp[~keep] *= (lc.rmax * 0.999 / rad[~keep])[:, None]  # pull the few escapees inside
+    perm = rng.permutation(n)
+    out["Pos"] = allp[perm].astype(np.float32)
+    out["Vel"] = rng.normal(0, 300, (n, 3)).astype(np.float32)
+    out["ID"] = perm
+    return out
+
+
+def lightcone_shells(lc: Lightcone, P: np.ndarray):
+    """Radial shells with radial ghost layers as one batch.  Shell k owns
+    r in [r_k, r_k+1); its domain is the cube around the origin that holds
+    r_k+1 + ghost, with lcell = the mean separation at r_k+1 (so b <= 0.2
+    inside the shell).  Returns (records, volumes, owned-radius bounds)."""
+    edges = lc.shells()
+    r = np.linalg.norm(P["Pos"].astype(np.float64), axis=1)
+    g = lc.ghost
+    r2max, tab = lc.varlink_table()
+    vt = L.VarLinkTable([0.0, 0.0, 0.0], r2max, tab)
+    parts, vols, bounds, off = [], [], [], 0
+    for k in range(lc.nshell):
+        lo, hi = edges[k], edges[k + 1]
+        sel = (r >= lo - g) & (r < hi + g)
+        Pk = P[sel]
+        R = hi + g
+        sep = float(lc.density(hi) ** (-1.0 / 3.0))
+        v = L.Volume()
+        v.offset, v.n = off, len(Pk)
+        v.domain = L.domain([-R] * 3, [R] * 3, [-R] * 3, [R] * 3, [sep] * 3, float(lc.link(hi)),
+                            lc.nphalomin, vt)
+        off += len(Pk)
+        parts.append(Pk)
+        vols.append(v)
+        bounds.append((lo, hi))
+    return np.concatenate(parts), vols, bounds, vt

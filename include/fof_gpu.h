@@ -129,26 +129,35 @@ typedef struct fof_halo_v1 {
   double sum_vel[3];
 } fof_halo_v1;
 
-/* Counters and timings. */
-#define FOF_NPHASE 8
+/* Counters and timings.  With FOF_OPT_TIMING, phase_ms[k] is the device time
+ * (HIP events on the library's stream) of span k of the last call, 0 when the
+ * span did not run:
+ *   0 cell keys (k_keys)            1 cell starts (scan) or key sort (sparse)
+ *   2 counting-sort scatter         3 union-find init
+ *   4 sub-cell sort of dense cells  5 pair tests, sparse cells
+ *   6 pair tests, dense cells       7 union-find flatten + group minima
+ *   8 labels                        9 compaction of kept groups
+ *  10 radix sort by label          11 ring
+ *  12 halo list                    13 halo sums
+ *  14 output copies                15 total, first span to last            */
+#define FOF_NPHASE 16
 typedef struct fof_stats_v1 {
   uint32_t version;
   uint32_t nvol;
   uint64_t n;             /* particles linked                                */
-  uint64_t ncells;        /* cells in the grids used                         */
+  uint64_t ncells;        /* cells in the grids used (occupied cells when
+                             sparse)                                         */
   uint64_t ndense;        /* cells linked at sub-cell level                  */
   uint64_t pair_tests;    /* particle-pair distance tests                    */
   uint64_t rechecks;      /* tests re-decided in FP64                        */
   uint64_t ngroups;       /* groups of any size                              */
-  uint64_t nmembers;      /* particles in groups of size >= 2                */
+  uint64_t nmembers;      /* particles in groups kept for the ring sort      */
   uint64_t nhalos;        /* groups passing the size cut                     */
   uint64_t nhalos_owned;  /* of those, owned (Checkhalo)                     */
-  double phase_ms[FOF_NPHASE]; /* with FOF_OPT_TIMING, device time of: 0 cell
-                               keys + counting sort, 1 sub-cell sort, 2 pair
-                               tests (both link kernels), 3 union-find finish
-                               + labels, 4 compaction + ring sort, 5 ring +
-                               halo sums, 6 output copies, 7 total           */
-  double link_kernel_ms;  /* phase 2: the pair-test kernels' duration         */
+  uint32_t sparse;        /* 1 if occupied-cell addressing was used          */
+  uint32_t reserved;
+  double phase_ms[FOF_NPHASE];
+  double link_kernel_ms;  /* spans 5 + 6: the pair-test kernels              */
 } fof_stats_v1;
 
 typedef struct fof_ctx fof_ctx;

@@ -332,3 +332,23 @@ def test_sparse_batch(ctx):
         a, b = int(vol.offset), int(vol.offset + vol.n)
         olab, _ = O.link_uf(P["Pos"][a:b], odomain(vol))
         assert np.array_equal(outs[1][0][a:b], olab)
+
+
+@pytest.mark.parametrize("flags", [0, L.OPT_SPARSE])
+def test_lightcone_shells(ctx, flags):
+    lc = W.Lightcone(n=150000, nshell=8, seed=3)
+    P0 = W.lightcone_particles(lc)
+    P, vols, bounds, vt = W.lightcone_shells(lc, P0)
+    n = len(P)
+    lab = np.empty(n, np.uint32)
+    halos = np.empty(n, L.HALO_DTYPE)
+    hoff = np.zeros(len(vols) + 1, np.uint64)
+    st = L.Stats()
+    L.check(ctx.link_batch(L.particles(P), vols, L.opts(flags), None, lab, halos, n, hoff, st))
+    assert st.ncells > 0
+    r2max, tab = lc.varlink_table()
+    ovl = O.varlink([0.0, 0.0, 0.0], r2max, tab)
+    for v, vol in enumerate(vols):
+        a, b = int(vol.offset), int(vol.offset + vol.n)
+        olab, _ = O.link_uf(P["Pos"][a:b], odomain(vol), ovl)
+        assert np.array_equal(lab[a:b], olab), v

@@ -101,21 +101,35 @@ def volume_box(k: int, n: int, box: float, b: float, seed: int = 5, shell: float
     return bx
 
 
-def volumes_host(nvol: int, median: float, lo: float, hi: float, seed: int = 5):
-    """Independent periodic volumes built on the host (scaled C5)."""
+def _one_volume(k, n, box, b, seed):
+    bx = volume_box(k, int(n), float(box), float(b), seed)
+    g = L.gen_params(bx.seed, int(n), bx.box, max(1, int(round(n * CLUMPS_PER_PARTICLE))),
+                     pair_l=bx.linklength)
+    recs = L.gen_host(g)
+    gh = L.ghost_params(bx.box, bx.shell, 1, 1, 1, 1, bx.linklength, bx.nphalomin)
+    out, counts, vv = L.ghosts_host(gh, recs)
+    v = vv[0]
+    # the reference cell of a periodic volume of n particles: box / n^(1/3)
+    for d in range(3):
+        v.domain.lcell[d] = bx.box / np.cbrt(float(n))
+    return out, v
+
+
+def volumes_host(nvol: int, median: float = 6e4, lo: float = 1e3, hi: float = 2e6, seed: int = 5,
+                 workers: int = 1):
+    """Independent periodic volumes (config C5 at full size with the
+    defaults), each with its periodic shell, built on the host; the C
+    generator releases the GIL, so ``workers`` threads run in parallel."""
     n, boxes, b = volumes_spec(nvol, median, lo=lo, hi=hi, seed=seed)
+    if workers > 1:
+        import concurrent.futures as cf
+
+        with cf.ThreadPoolExecutor(workers) as ex:
+            res = list(ex.map(lambda k: _one_volume(k, n[k], boxes[k], b[k], seed), range(nvol)))
+    else:
+        res = [_one_volume(k, n[k], boxes[k], b[k], seed) for k in range(nvol)]
     parts, vols, off = [], [], 0
-    for k in range(nvol):
-        bx = volume_box(k, int(n[k]), float(boxes[k]), float(b[k]), seed)
-        g = L.gen_params(bx.seed, int(n[k]), bx.box, max(1, int(round(n[k] * CLUMPS_PER_PARTICLE))),
-                         pair_l=bx.linklength)
-        recs = L.gen_host(g)
-        gh = L.ghost_params(bx.box, bx.shell, 1, 1, 1, 1, bx.linklength, bx.nphalomin)
-        out, counts, vv = L.ghosts_host(gh, recs)
-        v = vv[0]
-        # the reference cell for a volume of n particles: box / n^(1/3)
-        for d in range(3):
-            v.domain.lcell[d] = bx.box / np.cbrt(float(n[k]))
+    for out, v in res:
         v.offset = off
         off += len(out)
         parts.append(out)

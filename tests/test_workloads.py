@@ -76,3 +76,29 @@ def test_ghost_shell_semantics():
             key = np.rec.fromarrays([P["ID"], P["Pos"][:, 0], P["Pos"][:, 1], P["Pos"][:, 2]])
             assert len(np.unique(key)) == len(P)
         assert owned_total == len(recs)  # each particle owned exactly once
+
+
+def test_comoving_distance_closed_form():
+    # Einstein-de Sitter (om = 1): r(z) = 2 c/H0 (1 - 1/sqrt(1+z))
+    z, r = W.comoving_distance_table(om=1.0, zmax=3.0, nz=30001)
+    exact = 2 * W.C_OVER_H0 * (1 - 1 / np.sqrt(1 + z))
+    assert np.abs(r - exact).max() < 1e-6 * exact.max()
+
+
+def test_lightcone_density_and_shells():
+    lc = W.Lightcone(n=100000, nshell=8, seed=2)
+    rr = np.linspace(0, lc.rmax, 20001)
+    integral = np.trapz(4 * np.pi * rr ** 2 * lc.density(rr), rr)
+    assert abs(integral / lc.n - 1) < 1e-4
+    assert lc.z_of_r(0.0) == 0.0 and 1.4 < lc.z_of_r(3000.0) < 1.55  # flat LCDM, om 0.31
+    P = W.lightcone_particles(lc)
+    r = np.linalg.norm(P["Pos"].astype(np.float64), axis=1)
+    assert r.max() < lc.rmax and np.array_equal(np.sort(P["ID"]), np.arange(lc.n))
+    Q, vols, bounds, vt = W.lightcone_shells(lc, P)
+    owned = 0
+    for v, (lo, hi) in zip(vols, bounds):
+        rq = np.linalg.norm(Q["Pos"][int(v.offset):int(v.offset + v.n)].astype(np.float64), axis=1)
+        assert rq.min() >= lo - lc.ghost and rq.max() < hi + lc.ghost
+        owned += ((rq >= lo) & (rq < hi)).sum()
+        assert v.domain.linklength < v.domain.lcell[0]  # l < lcell inside every shell
+    assert owned == lc.n

@@ -47,17 +47,26 @@ def mix_ids(ids):
     return z ^ (z >> 32)
 
 
-def group_fingerprints(recs_u8, n, label, halos_u8, nh):
-    """(size, sum of mixed member IDs, xor-free) per owned halo, sorted."""
+def group_fingerprints(recs_u8, n, label, halos_u8, nh, vol_off=None, vol_n=None, halo_off=None):
+    """(size, sum of mixed member IDs) of every owned halo, sorted: an
+    order-free fingerprint of group membership.  Labels and heads are
+    volume-local in a batch; vol_off / vol_n / halo_off map them back."""
     import torch
 
+    dev = label.device
     ids = recs_u8[: n * 32].view(torch.int64).view(n, 4)[:, 0]
     lab = label[:n].to(torch.int64) & 0xFFFFFFFF
     h = halos_u8[: nh * 64].view(torch.int32).view(nh, 16)
     head = h[:, 0].to(torch.int64) & 0xFFFFFFFF
+    if vol_off is not None:
+        off = torch.tensor(np.asarray(vol_off, np.int64), device=dev)
+        cnt = torch.tensor(np.asarray(vol_n, np.int64), device=dev)
+        lab = lab + torch.repeat_interleave(off, cnt)
+        hcnt = torch.tensor(np.diff(np.asarray(halo_off, np.int64)), device=dev)
+        head = head + torch.repeat_interleave(off, hcnt)
     size = h[:, 1].to(torch.int64)
     owned = h[:, 2] == 1
-    acc = torch.zeros(n, dtype=torch.int64, device=label.device)
+    acc = torch.zeros(n, dtype=torch.int64, device=dev)
     acc.index_add_(0, lab, mix_ids(ids))
     fp = torch.stack([size[owned], acc[head[owned]]], 1)
     fp = fp[torch.argsort(fp[:, 1])]
@@ -148,6 +157,8 @@ def main():
                          "with 1%% of particles in borderline pairs; volumes: independent "
                          "periodic volumes with log-normal counts")
     ap.add_argument("--nvol", type=int, default=4096)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 4 GiB device-to-device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the FoF domains shard "
@@ -163,6 +174,15 @@ def main():
 
     dev = torch.device("cuda:0")
     log("device:", L.device_info(0))
+    if args.copy_probe:
+        a = torch.empty(1 << 32, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 4 GiB device-to-device copy done")
+        del a, b
     if args.workload == "volumes":
         return run_volumes(args, torch, L, W, O)
     if args.workload == "precision":
@@ -216,7 +236,8 @@ def main():
     t_batch = run_steps(step_batch, args.warmup, args.steps)
     log(f"batched: {t_batch * 1e3:.1f} ms/step  phases {dict(zip(L.PHASES, [round(x, 2) for x in stB.phase_ms]))}")
     nhB = int(hoff[-1])
-    fpB = group_fingerprints(DB, NB, lab, hal, nhB)
+    fpB = group_fingerprints(DB, NB, lab, hal, nhB, [int(v.offset) for v in VB], [int(v.n) for v in VB],
+                             hoff)
     same_groups = bool(fp1.shape == fpB.shape and torch.equal(fp1, fpB))
     log(f"owned halos plain {fp1.shape[0]} batched {fpB.shape[0]} identical membership: {same_groups}")
 

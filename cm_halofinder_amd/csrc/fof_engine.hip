@@ -548,9 +548,18 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
         nr++;
       }
     }
+    const bool subcells = V.m[0] > 0;
     for (int r = 0; r < nr; r++) {
       uint32_t j = r0[r];
       const uint32_t je = r1[r];
+      if (!subcells || je - j < T) {
+        // fewer than T particles: no dense cell here, every particle is a run
+        for (; j < je; j++) {
+          tests++;
+          if (pair_linked(px, pos4[j], R, rechecks)) uf_union(par, x, j);
+        }
+        continue;
+      }
       while (j < je) {
         const uint32_t re = run_end[j];
         if (re == j + 1) {
@@ -627,6 +636,48 @@ __device__ int collect_runs(const float4 *__restrict__ pos4,
   return (int)n;
 }
 
+// Union-find over run slots in LDS (A's runs 0..na-1, the neighbour's runs
+// after them).  Hooking larger-under-smaller keeps A's roots inside A, so the
+// neighbour's slots can be reset for the next neighbour.
+__device__ __forceinline__ uint32_t lfind(volatile uint32_t *lp, uint32_t x) {
+  for (;;) {
+    const uint32_t p = lp[x];
+    if (p == x) return x;
+    const uint32_t gp = lp[p];
+    if (gp == p) return p;
+    lp[x] = gp;
+    x = gp;
+  }
+}
+
+__device__ __forceinline__ void lunion(volatile uint32_t *lp, uint32_t a, uint32_t b) {
+  for (;;) {
+    a = lfind(lp, a);
+    b = lfind(lp, b);
+    if (a == b) return;
+    if (a > b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS((uint32_t *)&lp[b], b, a) == b) return;
+  }
+}
+
+// Any linking pair between two runs (both cliques)?  Early exit.
+__device__ __forceinline__ bool runs_link(const float4 *__restrict__ pos4, uint32_t ha,
+                                          uint32_t ea, uint32_t hb, uint32_t eb, const Rule &R,
+                                          uint32_t &tests, uint32_t &rechecks) {
+  for (uint32_t i = ha; i < ea; i++) {
+    const float4 pi = pos4[i];
+    for (uint32_t j = hb; j < eb; j++) {
+      tests++;
+      if (pair_linked(pi, pos4[j], R, rechecks)) return true;
+    }
+  }
+  return false;
+}
+
 __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__ dense,
                                                     const uint32_t *__restrict__ ndense,
                                                     const float4 *__restrict__ pos4,
@@ -640,6 +691,8 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
                                                     uint32_t *__restrict__ overflow) {
   __shared__ uint32_t ahead[kMaxRuns], aend[kMaxRuns], arel[kMaxRuns];
   __shared__ uint32_t bhead[kMaxRuns], bend[kMaxRuns], brel[kMaxRuns];
+  __shared__ uint32_t lp[2 * kMaxRuns];
+  __shared__ uint16_t amap[kMaxBins];
   __shared__ uint16_t aidx[kMaxRuns];
   __shared__ uint32_t cnt, acnt;
   uint32_t tests = 0, rechecks = 0;
@@ -654,93 +707,125 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
     const Rule R = make_rule(V, ix, iy, iz);
     const int mm[3] = {V.m[0], V.m[1], V.m[2]};
     const int ox = ix * mm[0], oy = iy * mm[1], oz = iz * mm[2];
-    // the home cell's runs, once
+    const int nbins = mm[0] * mm[1] * mm[2];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = 0xFFFF;
+    // the home cell's runs (one per occupied sub-cell), once
     const int na = collect_runs(pos4, run_end, sA, eA, V, ox, oy, oz, -1000, 1000, -1000, 1000, -1000,
                                 1000, ahead, aend, arel, &cnt);
     if (na > kMaxRuns) {
       if (threadIdx.x == 0) atomicOr(overflow, 1u);
       continue;
     }
+    for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
+      const uint32_t r = arel[ia];
+      amap[(rel_z(r) * mm[1] + rel_y(r)) * mm[0] + rel_x(r)] = (uint16_t)ia;
+      lp[ia] = ia;
+    }
+    __syncthreads();
     int reach[3];
     for (int a = 0; a < 3; a++)  // largest sub-cell offset whose gap can be within reach
       reach[a] = min((int)floor(sqrt(R.prune2 / V.s2[a])) + 1, mm[a]);
-    // 14 home-cell pairs: (A,A) and the 13 forward cells of src_v2/FOF.c:122-153
-    for (int q = 0; q < 14; q++) {
-      int dd[3] = {0, 0, 0};
-      if (q > 0) {
-        const int k = q - 1;  // 0..12
-        if (k == 0) dd[0] = 1;
-        else if (k < 4) dd[1] = 1, dd[0] = k - 2;
-        else dd[2] = 1, dd[1] = (k - 4) / 3 - 1, dd[0] = (k - 4) % 3 - 1;
+    // (A, A): forward sub-cell offsets within reach, through the sub-cell map
+    for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
+      const uint32_t ra = arel[ia];
+      const int ax = rel_x(ra), ay = rel_y(ra), az = rel_z(ra);
+      const uint32_t ha = ahead[ia], ea = aend[ia];
+      for (int dz = 0; dz <= reach[2]; dz++) {
+        const int bz = az + dz;
+        if (bz >= mm[2]) break;
+        const int gz = max(dz - 1, 0);
+        for (int dy = -reach[1]; dy <= reach[1]; dy++) {
+          if (dz == 0 && dy < 0) continue;
+          const int by = ay + dy;
+          if (by < 0 || by >= mm[1]) continue;
+          const int gy = max(abs(dy) - 1, 0);
+          for (int dx = -reach[0]; dx <= reach[0]; dx++) {
+            if (dz == 0 && dy == 0 && dx <= 0) continue;
+            const int bxx = ax + dx;
+            if (bxx < 0 || bxx >= mm[0]) continue;
+            const int gx = max(abs(dx) - 1, 0);
+            const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
+                              (double)(gz * gz) * V.s2[2];
+            if (g2 > R.prune2) continue;
+            const uint32_t ib = amap[(bz * mm[1] + by) * mm[0] + bxx];
+            if (ib == 0xFFFF) continue;
+            if (lfind(lp, ia) == lfind(lp, ib)) continue;
+            if (runs_link(pos4, ha, ea, ahead[ib], aend[ib], R, tests, rechecks))
+              lunion(lp, ia, ib);
+          }
+        }
       }
+    }
+    __syncthreads();
+    // (A, B) for the 13 forward cells of src_v2/FOF.c:122-153
+    for (int q = 1; q < 14; q++) {
+      int dd[3] = {0, 0, 0};
+      const int k = q - 1;  // 0..12
+      if (k == 0) dd[0] = 1;
+      else if (k < 4) dd[1] = 1, dd[0] = k - 2;
+      else dd[2] = 1, dd[1] = (k - 4) / 3 - 1, dd[0] = (k - 4) % 3 - 1;
       const int bx = ix + dd[0], by = iy + dd[1], bz = iz + dd[2];
       if (bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz >= V.nz) continue;
       const uint32_t cb = cell_index(G, V, bx, by, bz);
       if (cb == kNone) continue;
       const uint32_t sB = G.start[cb], eB = G.start[cb + 1];
       if (sB == eB) continue;
-      int n1 = na, n2 = na;
-      const uint32_t *h2 = ahead, *e2 = aend, *r2 = arel;
-      if (q > 0) {
-        // only runs within reach of the shared face / edge / corner
-        int alo[3], ahi[3], blo[3], bhi[3];
-        for (int a = 0; a < 3; a++) {
-          alo[a] = -1000, ahi[a] = 1000, blo[a] = -1000, bhi[a] = 1000;
-          if (dd[a] == 1) alo[a] = mm[a] - reach[a], blo[a] = mm[a], bhi[a] = mm[a] - 1 + reach[a];
-          else if (dd[a] == -1) ahi[a] = reach[a] - 1, blo[a] = -reach[a], bhi[a] = -1;
-        }
-        n2 = collect_runs(pos4, run_end, sB, eB, V, ox, oy, oz, blo[0], bhi[0], blo[1], bhi[1], blo[2],
-                          bhi[2], bhead, bend, brel, &cnt);
-        if (n2 > kMaxRuns) {
-          if (threadIdx.x == 0) atomicOr(overflow, 1u);
-          continue;
-        }
-        if (threadIdx.x == 0) acnt = 0;
-        __syncthreads();
-        for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
-          const uint32_t r = arel[ia];
-          const int rx = rel_x(r), ry = rel_y(r), rz = rel_z(r);
-          if (rx >= alo[0] && rx <= ahi[0] && ry >= alo[1] && ry <= ahi[1] && rz >= alo[2] &&
-              rz <= ahi[2])
-            aidx[atomicAdd(&acnt, 1u)] = (uint16_t)ia;
-        }
-        __syncthreads();
-        n1 = (int)acnt;
-        __syncthreads();
-        h2 = bhead, e2 = bend, r2 = brel;
+      // only runs within reach of the shared face / edge / corner
+      int alo[3], ahi[3], blo[3], bhi[3];
+      for (int a = 0; a < 3; a++) {
+        alo[a] = -1000, ahi[a] = 1000, blo[a] = -1000, bhi[a] = 1000;
+        if (dd[a] == 1) alo[a] = mm[a] - reach[a], blo[a] = mm[a], bhi[a] = mm[a] - 1 + reach[a];
+        else if (dd[a] == -1) ahi[a] = reach[a] - 1, blo[a] = -reach[a], bhi[a] = -1;
       }
-      if (n1 == 0 || n2 == 0) continue;
-      for (int t = threadIdx.x; t < n1; t += blockDim.x) {
-        const int ia = (q == 0) ? t : (int)aidx[t];
-        const uint32_t ra = arel[ia];
-        const int ax = rel_x(ra), ay = rel_y(ra), az = rel_z(ra);
-        const uint32_t ha = ahead[ia], ea = aend[ia];
-        for (int ib = (q == 0) ? ia + 1 : 0; ib < n2; ib++) {
-          const uint32_t rb = r2[ib];
-          const int gx = max(abs(rel_x(rb) - ax) - 1, 0);
-          const int gy = max(abs(rel_y(rb) - ay) - 1, 0);
-          const int gz = max(abs(rel_z(rb) - az) - 1, 0);
-          const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
-                            (double)(gz * gz) * V.s2[2];
-          if (g2 > R.prune2) continue;
-          const uint32_t hb = h2[ib];
-          if (uf_find(par, ha) == uf_find(par, hb)) continue;
-          const uint32_t eb = e2[ib];
-          bool linked = false;
-          for (uint32_t i = ha; i < ea && !linked; i++) {
-            const float4 pi = pos4[i];
-            for (uint32_t j = hb; j < eb; j++) {
-              tests++;
-              if (pair_linked(pi, pos4[j], R, rechecks)) {
-                linked = true;
-                break;
-              }
-            }
-          }
-          if (linked) uf_union(par, ha, hb);
+      const int nb = collect_runs(pos4, run_end, sB, eB, V, ox, oy, oz, blo[0], bhi[0], blo[1],
+                                  bhi[1], blo[2], bhi[2], bhead, bend, brel, &cnt);
+      if (nb > kMaxRuns) {
+        if (threadIdx.x == 0) atomicOr(overflow, 1u);
+        continue;
+      }
+      if (nb == 0) continue;
+      if (threadIdx.x == 0) acnt = 0;
+      for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) lp[na + ib] = na + ib;
+      __syncthreads();
+      for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
+        const uint32_t r = arel[ia];
+        const int rx = rel_x(r), ry = rel_y(r), rz = rel_z(r);
+        if (rx >= alo[0] && rx <= ahi[0] && ry >= alo[1] && ry <= ahi[1] && rz >= alo[2] &&
+            rz <= ahi[2])
+          aidx[atomicAdd(&acnt, 1u)] = (uint16_t)ia;
+      }
+      __syncthreads();
+      const int n1 = (int)acnt;
+      // pairs (filtered A run, B run), B-major so threads share A's roots
+      const int npairs = n1 * nb;
+      for (int t = threadIdx.x; t < npairs; t += blockDim.x) {
+        const int ib = t / n1, ia = aidx[t - ib * n1];
+        const uint32_t ra = arel[ia], rb = brel[ib];
+        const int gx = max(abs(rel_x(rb) - rel_x(ra)) - 1, 0);
+        const int gy = max(abs(rel_y(rb) - rel_y(ra)) - 1, 0);
+        const int gz = max(abs(rel_z(rb) - rel_z(ra)) - 1, 0);
+        const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
+                          (double)(gz * gz) * V.s2[2];
+        if (g2 > R.prune2) continue;
+        if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
+        if (runs_link(pos4, ahead[ia], aend[ia], bhead[ib], bend[ib], R, tests, rechecks))
+          lunion(lp, ia, na + ib);
+      }
+      __syncthreads();
+      // commit the neighbour's runs: each joins its LDS root's run globally
+      for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
+        const uint32_t r = lfind(lp, na + ib);
+        if (r != (uint32_t)(na + ib)) {
+          const uint32_t hr = (r < (uint32_t)na) ? ahead[r] : bhead[r - na];
+          uf_union(par, bhead[ib], hr);
         }
       }
       __syncthreads();
+    }
+    // commit the home cell's runs
+    for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
+      const uint32_t r = lfind(lp, ia);
+      if (r != (uint32_t)ia) uf_union(par, ahead[ia], ahead[r]);
     }
     __syncthreads();
   }
@@ -750,6 +835,9 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
 // ---------------------------------------------------------------------------
 // 7: union-find finish
 // ---------------------------------------------------------------------------
+// Flatten every particle to its root; the other members of a group
+// accumulate their minimum input index and their count at the root (minorig
+// and gsize start at ~0 and 0; the root itself is added in k_label).
 __global__ __launch_bounds__(256) void k_flatten(uint32_t N, uint32_t *__restrict__ par,
                                                  const float4 *__restrict__ pos4,
                                                  uint32_t *__restrict__ minorig,
@@ -763,29 +851,18 @@ __global__ __launch_bounds__(256) void k_flatten(uint32_t N, uint32_t *__restric
       r = p;
       p = par[r];
     }
-    par[x] = r;
-    minorig[x] = __float_as_uint(pos4[x].w);
-    gsize[x] = 1;
-    roots += (r == x);
+    if (r != x) {
+      par[x] = r;
+      atomicMin(minorig + r, __float_as_uint(pos4[x].w));
+      atomicAdd(gsize + r, 1u);
+    } else {
+      roots++;
+    }
   }
   uint64_t t = roots;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
   if ((threadIdx.x & 63) == 0 && t) atomicAdd(ctr + 2, (unsigned long long)t);
-}
-
-__global__ __launch_bounds__(256) void k_gather(uint32_t N, const uint32_t *__restrict__ par,
-                                                const float4 *__restrict__ pos4,
-                                                uint32_t *__restrict__ minorig,
-                                                uint32_t *__restrict__ gsize) {
-  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
-       x += gridDim.x * blockDim.x) {
-    const uint32_t r = par[x];
-    if (r != x) {
-      atomicMin(minorig + r, __float_as_uint(pos4[x].w));
-      atomicAdd(gsize + r, 1u);
-    }
-  }
 }
 
 // label[orig] = group minimum index (global), with kFlag when the group is
@@ -802,8 +879,9 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
   for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
        x += gridDim.x * blockDim.x) {
     const uint32_t r = par[x];
-    const uint32_t lab = minorig[r], sz = gsize[r];
     const uint32_t o = __float_as_uint(pos4[x].w);
+    const uint32_t orr = (r == x) ? o : __float_as_uint(pos4[r].w);
+    const uint32_t lab = min(minorig[r], orr), sz = gsize[r] + 1u;
     const VolDev &V = vols[vol_of(vbeg, nvol, x)];
     const bool keep = (sz >= 2u) || !((double)sz < V.nphalomin);
     label[o] = lab | (keep ? kFlag : 0u);
@@ -853,7 +931,8 @@ __global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
 // Halo sums: FP64 accumulation of the float Pos / Vel of the members in
 // ascending index order from the head (the order the host walks the ring
 // returned in next_out), so the host's Checkhalo / Output_Halos loops give
-// the same bits.  Loads are issued 16 ahead of the ordered accumulation.
+// the same bits.  One wavefront per halo: the 64 lanes gather 64 members'
+// records at a time into LDS and lane 0 adds them in order.
 __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ dH,
                                                    const uint32_t *__restrict__ hk,
                                                    const uint32_t *__restrict__ skey,
@@ -867,49 +946,53 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
                                                    fof_halo_v1 *__restrict__ out,
                                                    uint32_t *__restrict__ hhead,
                                                    unsigned long long *__restrict__ ctr) {
+  __shared__ float stage[4][6][64];
   const uint32_t H = *dH;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t nw = gridDim.x * (blockDim.x >> 6);
   uint32_t owned_cnt = 0;
-  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H;
-       h += gridDim.x * blockDim.x) {
+  for (uint32_t h = blockIdx.x * (blockDim.x >> 6) + w; h < H; h += nw) {
     const uint32_t k0 = hk[h];
     const uint32_t lab = skey[k0];
     const uint32_t n = hsize[lab];
     const int v = vol_of(vbeg, nvol, lab);
     const VolDev &V = vols[v];
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    constexpr int kAhead = 16;
-    for (uint32_t k = 0; k < n; k += kAhead) {
-      float a[kAhead][6];
-      const uint32_t m = min((uint32_t)kAhead, n - k);
-#pragma unroll
-      for (int q = 0; q < kAhead; q++) {
-        if ((uint32_t)q < m) {
-          const uint8_t *rec = base + (uint64_t)sval[k0 + k + q] * stride;
-          const float *pp = reinterpret_cast<const float *>(rec + offpos);
-          const float *vv = reinterpret_cast<const float *>(rec + offvel);
-          a[q][0] = pp[0], a[q][1] = pp[1], a[q][2] = pp[2];
-          a[q][3] = vv[0], a[q][4] = vv[1], a[q][5] = vv[2];
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    for (uint32_t k = 0; k < n; k += 64) {
+      const uint32_t m = min(64u, n - k);
+      if ((uint32_t)lane < m) {
+        const uint8_t *rec = base + (uint64_t)sval[k0 + k + lane] * stride;
+        const float *pp = reinterpret_cast<const float *>(rec + offpos);
+        const float *vv = reinterpret_cast<const float *>(rec + offvel);
+        stage[w][0][lane] = pp[0], stage[w][1][lane] = pp[1], stage[w][2][lane] = pp[2];
+        stage[w][3][lane] = vv[0], stage[w][4][lane] = vv[1], stage[w][5][lane] = vv[2];
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (lane == 0) {
+        for (uint32_t q = 0; q < m; q++) {
+          s0 += (double)stage[w][0][q];
+          s1 += (double)stage[w][1][q];
+          s2 += (double)stage[w][2][q];
+          s3 += (double)stage[w][3][q];
+          s4 += (double)stage[w][4][q];
+          s5 += (double)stage[w][5][q];
         }
       }
-#pragma unroll
-      for (int q = 0; q < kAhead; q++) {
-        if ((uint32_t)q < m) {
-#pragma unroll
-          for (int t = 0; t < 6; t++) s[t] += (double)a[q][t];
-        }
-      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
-    fof_halo_v1 H1;
-    H1.head = lab - vbeg[v];
-    H1.n = n;
+    // broadcast lane 0's sums
+    s0 = __shfl(s0, 0, 64), s1 = __shfl(s1, 0, 64), s2 = __shfl(s2, 0, 64);
+    s3 = __shfl(s3, 0, 64), s4 = __shfl(s4, 0, 64), s5 = __shfl(s5, 0, 64);
     // src_v2/FOF.c:394-401
-    const double cx = s[0] / (double)n, cy = s[1] / (double)n, cz = s[2] / (double)n;
+    const double cx = s0 / (double)n, cy = s1 / (double)n, cz = s2 / (double)n;
     const bool own = !((cx < V.rmin[0]) || (cx >= V.rmax[0]) || (cy < V.rmin[1]) ||
                        (cy >= V.rmax[1]) || (cz < V.rmin[2]) || (cz >= V.rmax[2]));
     uint32_t warn = 0;
     if (own) {
       // src_v2/FOF.c:403-422
-      for (uint32_t k = 0; k < n; k++) {
+      for (uint32_t k = lane; k < n; k += 64) {
         const float *pp =
             reinterpret_cast<const float *>(base + (uint64_t)sval[k0 + k] * stride + offpos);
 #pragma unroll
@@ -919,16 +1002,21 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
             warn |= 1u << d;
         }
       }
-      owned_cnt++;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) warn |= __shfl_xor(warn, o, 64);
     }
-    H1.owned = own ? 1u : 0u;
-    H1.edge_warn = warn;
-    for (int t = 0; t < 3; t++) {
-      H1.sum_pos[t] = s[t];
-      H1.sum_vel[t] = s[3 + t];
+    if (lane == 0) {
+      fof_halo_v1 H1;
+      H1.head = lab - vbeg[v];
+      H1.n = n;
+      H1.owned = own ? 1u : 0u;
+      H1.edge_warn = warn;
+      H1.sum_pos[0] = s0, H1.sum_pos[1] = s1, H1.sum_pos[2] = s2;
+      H1.sum_vel[0] = s3, H1.sum_vel[1] = s4, H1.sum_vel[2] = s5;
+      out[h] = H1;
+      hhead[h] = lab;
+      owned_cnt += own ? 1u : 0u;
     }
-    out[h] = H1;
-    hhead[h] = lab;
   }
   uint64_t t = owned_cnt;
 #pragma unroll
@@ -1376,9 +1464,9 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   // 7: union-find finish (minorig reuses runend, hsize reuses key)
   uint32_t *minorig = runend, *hsize = key;
   if ((rc = tb(7))) return rc;
+  FOF_HIP(hipMemsetAsync(minorig, 0xFF, N * 4, st));
+  FOF_HIP(hipMemsetAsync(gsize, 0, N * 4, st));
   k_flatten<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, ctr);
-  FOF_HIP(hipGetLastError());
-  k_gather<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize);
   FOF_HIP(hipGetLastError());
   if ((rc = te(7))) return rc;
   if ((rc = tb(8))) return rc;
@@ -1423,7 +1511,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
         hout = static_cast<fof_halo_v1 *>(W.out_halo.ptr);
       }
       if ((rc = tb(13))) return rc;
-      k_halo_sums<<<grid_for(H), 256, 0, st>>>(small + 3, sv1, sk0, sv0, hsize, dbase, stride,
+      k_halo_sums<<<grid_for(H * 64), 256, 0, st>>>(small + 3, sv1, sk0, sv0, hsize, dbase, stride,
                                                offpos, offvel, dV, dVB, (int)nvol, hout,
                                                static_cast<uint32_t *>(W.hhead.ptr), ctr);
       FOF_HIP(hipGetLastError());

@@ -250,6 +250,25 @@ __device__ __forceinline__ void uf_union(uint32_t *par, uint32_t a, uint32_t b) 
   }
 }
 
+// Join j to the set whose root (or an ancestor of it) is rx; returns the
+// current root.  Skips the hook when j is already in the set.
+__device__ __forceinline__ uint32_t uf_join(uint32_t *par, uint32_t rx, uint32_t j) {
+  uint32_t b = uf_find(par, j);
+  for (;;) {
+    uint32_t a = uf_find(par, rx);
+    if (a == b) return a;
+    if (a > b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    const uint32_t old = atomicCAS(par + b, b, a);
+    if (old == b) return a;
+    rx = a;
+    b = uf_find(par, old);
+  }
+}
+
 __device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t tests,
                                              uint32_t rechecks) {
   uint64_t t = tests, r = rechecks;
@@ -520,6 +539,7 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
     const uint32_t s = G.start[c], e = G.start[c + 1];
     if (V.m[0] > 0 && e - s >= T) continue;  // dense: k_link_dense
     const Rule R = make_rule(V, cp.ix, cp.iy, cp.iz);
+    uint32_t rx = x;  // x's root, refreshed by every join
     const int nx = V.nx, ny = V.ny, nz = V.nz;
     const int xl = (cp.ix > 0) ? 1 : 0, xr = (cp.ix + 1 < nx) ? 1 : 0;
     // Up to five contiguous ranges cover the own cell (after x) and the 13
@@ -554,9 +574,23 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
       const uint32_t je = r1[r];
       if (!subcells || je - j < T) {
         // fewer than T particles: no dense cell here, every particle is a run
+        for (; j + 8 <= je; j += 8) {
+          float4 q[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) q[k] = pos4[j + k];
+          uint32_t hit = 0;
+#pragma unroll
+          for (int k = 0; k < 8; k++) hit |= pair_linked(px, q[k], R, rechecks) ? (1u << k) : 0u;
+          tests += 8;
+          while (hit) {
+            const int k = __builtin_ctz(hit);
+            hit &= hit - 1;
+            rx = uf_join(par, rx, j + k);
+          }
+        }
         for (; j < je; j++) {
           tests++;
-          if (pair_linked(px, pos4[j], R, rechecks)) uf_union(par, x, j);
+          if (pair_linked(px, pos4[j], R, rechecks)) rx = uf_join(par, rx, j);
         }
         continue;
       }
@@ -564,7 +598,7 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
         const uint32_t re = run_end[j];
         if (re == j + 1) {
           tests++;
-          if (pair_linked(px, pos4[j], R, rechecks)) uf_union(par, x, j);
+          if (pair_linked(px, pos4[j], R, rechecks)) rx = uf_join(par, rx, j);
           j++;
           continue;
         }
@@ -576,11 +610,11 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
         const int gz = max(abs(cj.fz - cp.fz) - 1, 0);
         const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
                           (double)(gz * gz) * V.s2[2];
-        if (!(g2 > R.prune2) && uf_find(par, x) != uf_find(par, j)) {
+        if (!(g2 > R.prune2) && uf_find(par, rx) != uf_find(par, j)) {
           for (uint32_t k = j; k < re; k++) {
             tests++;
             if (pair_linked(px, pos4[k], R, rechecks)) {
-              uf_union(par, x, k);
+              rx = uf_join(par, rx, k);
               break;
             }
           }
@@ -595,7 +629,14 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
 // ---------------------------------------------------------------------------
 // 6: dense cells -- one workgroup per cell, run pairs with early exit.
 // ---------------------------------------------------------------------------
-constexpr int kMaxRuns = 2048;
+// Run-table capacities.  The home cell has one run per occupied sub-cell,
+// at most 12^3.  A neighbour contributes only runs within reach of the
+// shared face/edge/corner: at most reach*m*m <= 288 for a dense neighbour
+// (reach <= 2 when m >= 5, reach <= 4 and m <= 4 otherwise) and fewer than
+// T <= kMaxT particles for a sparse one.
+constexpr int kMaxRuns = 1728;
+constexpr int kMaxRunsB = 512;
+constexpr uint32_t kMaxT = 512;
 
 __device__ __forceinline__ uint32_t pack_rel(int a, int b, int c) {
   return ((uint32_t)(a & 255)) | ((uint32_t)(b & 255) << 8) | ((uint32_t)(c & 255) << 16);
@@ -612,7 +653,7 @@ __device__ int collect_runs(const float4 *__restrict__ pos4,
                             const uint32_t *__restrict__ run_end, uint32_t s, uint32_t e,
                             const VolDev &V, int ox, int oy, int oz, int wlo0, int whi0,
                             int wlo1, int whi1, int wlo2, int whi2, uint32_t *head,
-                            uint32_t *end, uint32_t *rel, uint32_t *cnt) {
+                            uint32_t *end, uint32_t *rel, uint32_t *cnt, uint32_t cap) {
   if (threadIdx.x == 0) *cnt = 0;
   __syncthreads();
   for (uint32_t j = s + threadIdx.x; j < e; j += blockDim.x) {
@@ -624,7 +665,7 @@ __device__ int collect_runs(const float4 *__restrict__ pos4,
     const int rx = c.fx - ox, ry = c.fy - oy, rz = c.fz - oz;
     if (rx < wlo0 || rx > whi0 || ry < wlo1 || ry > whi1 || rz < wlo2 || rz > whi2) continue;
     const uint32_t k = atomicAdd(cnt, 1u);
-    if (k < (uint32_t)kMaxRuns) {
+    if (k < cap) {
       head[k] = j;
       end[k] = re;
       rel[k] = pack_rel(rx, ry, rz);
@@ -664,13 +705,25 @@ __device__ __forceinline__ void lunion(volatile uint32_t *lp, uint32_t a, uint32
   }
 }
 
-// Any linking pair between two runs (both cliques)?  Early exit.
+// Any linking pair between two runs (both cliques)?  Early exit.  The
+// partner positions are loaded eight at a time so that the loads overlap.
 __device__ __forceinline__ bool runs_link(const float4 *__restrict__ pos4, uint32_t ha,
                                           uint32_t ea, uint32_t hb, uint32_t eb, const Rule &R,
                                           uint32_t &tests, uint32_t &rechecks) {
   for (uint32_t i = ha; i < ea; i++) {
     const float4 pi = pos4[i];
-    for (uint32_t j = hb; j < eb; j++) {
+    uint32_t j = hb;
+    for (; j + 8 <= eb; j += 8) {
+      float4 q[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) q[k] = pos4[j + k];
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < 8; k++) any |= pair_linked(pi, q[k], R, rechecks);
+      tests += 8;
+      if (any) return true;
+    }
+    for (; j < eb; j++) {
       tests++;
       if (pair_linked(pi, pos4[j], R, rechecks)) return true;
     }
@@ -690,10 +743,10 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
                                                     unsigned long long *__restrict__ ctr,
                                                     uint32_t *__restrict__ overflow) {
   __shared__ uint32_t ahead[kMaxRuns], aend[kMaxRuns], arel[kMaxRuns];
-  __shared__ uint32_t bhead[kMaxRuns], bend[kMaxRuns], brel[kMaxRuns];
-  __shared__ uint32_t lp[2 * kMaxRuns];
-  __shared__ uint16_t amap[kMaxBins];
-  __shared__ uint16_t aidx[kMaxRuns];
+  __shared__ uint32_t bhead[kMaxRunsB], bend[kMaxRunsB], brel[kMaxRunsB];
+  __shared__ uint32_t lp[kMaxRuns + kMaxRunsB];
+  __shared__ uint16_t amap[kMaxRuns];  // sub-cell -> run (A,A); filtered A runs (A,B)
+  uint16_t *const aidx = amap;
   __shared__ uint32_t cnt, acnt;
   uint32_t tests = 0, rechecks = 0;
   const uint32_t nd = *ndense;
@@ -708,10 +761,10 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
     const int mm[3] = {V.m[0], V.m[1], V.m[2]};
     const int ox = ix * mm[0], oy = iy * mm[1], oz = iz * mm[2];
     const int nbins = mm[0] * mm[1] * mm[2];
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = 0xFFFF;
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = 0xFFFF;  // nbins <= 1728
     // the home cell's runs (one per occupied sub-cell), once
     const int na = collect_runs(pos4, run_end, sA, eA, V, ox, oy, oz, -1000, 1000, -1000, 1000, -1000,
-                                1000, ahead, aend, arel, &cnt);
+                                1000, ahead, aend, arel, &cnt, kMaxRuns);
     if (na > kMaxRuns) {
       if (threadIdx.x == 0) atomicOr(overflow, 1u);
       continue;
@@ -750,11 +803,19 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
             const uint32_t ib = amap[(bz * mm[1] + by) * mm[0] + bxx];
             if (ib == 0xFFFF) continue;
             if (lfind(lp, ia) == lfind(lp, ib)) continue;
-            if (runs_link(pos4, ha, ea, ahead[ib], aend[ib], R, tests, rechecks))
+            // joined elsewhere (other workgroups, sparse-cell threads)?
+            if (uf_find(par, ha) == uf_find(par, ahead[ib]) ||
+                runs_link(pos4, ha, ea, ahead[ib], aend[ib], R, tests, rechecks))
               lunion(lp, ia, ib);
           }
         }
       }
+    }
+    __syncthreads();
+    // publish the home cell's merges before the neighbour passes
+    for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
+      const uint32_t r = lfind(lp, ia);
+      if (r != (uint32_t)ia) uf_union(par, ahead[ia], ahead[r]);
     }
     __syncthreads();
     // (A, B) for the 13 forward cells of src_v2/FOF.c:122-153
@@ -778,8 +839,8 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
         else if (dd[a] == -1) ahi[a] = reach[a] - 1, blo[a] = -reach[a], bhi[a] = -1;
       }
       const int nb = collect_runs(pos4, run_end, sB, eB, V, ox, oy, oz, blo[0], bhi[0], blo[1],
-                                  bhi[1], blo[2], bhi[2], bhead, bend, brel, &cnt);
-      if (nb > kMaxRuns) {
+                                  bhi[1], blo[2], bhi[2], bhead, bend, brel, &cnt, kMaxRunsB);
+      if (nb > kMaxRunsB) {
         if (threadIdx.x == 0) atomicOr(overflow, 1u);
         continue;
       }
@@ -808,7 +869,8 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
                           (double)(gz * gz) * V.s2[2];
         if (g2 > R.prune2) continue;
         if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
-        if (runs_link(pos4, ahead[ia], aend[ia], bhead[ib], bend[ib], R, tests, rechecks))
+        if (uf_find(par, ahead[ia]) == uf_find(par, bhead[ib]) ||
+            runs_link(pos4, ahead[ia], aend[ia], bhead[ib], bend[ib], R, tests, rechecks))
           lunion(lp, ia, na + ib);
       }
       __syncthreads();
@@ -1148,7 +1210,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   const uint32_t flags = opts ? opts->flags : 0u;
   double band = (opts && opts->band > 0) ? opts->band : 1e-5;
   if (!(band >= 1e-6 && band <= 0.1)) return FOF_E_ARG;
-  const uint32_t T = (opts && opts->dense_threshold) ? opts->dense_threshold : 12u;
+  const uint32_t T = std::min(kMaxT, (opts && opts->dense_threshold) ? opts->dense_threshold : 12u);
   const bool out_dev = (flags & FOF_OPT_OUT_DEVICE) != 0;
   const bool want_ring = next_out && !(flags & FOF_OPT_NO_RING);
   const bool want_halos = !(flags & FOF_OPT_NO_HALOS);

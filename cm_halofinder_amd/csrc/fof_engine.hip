@@ -82,6 +82,15 @@ __device__ __forceinline__ CellPos cell_pos(const VolDev &V, float x, float y, f
   return c;
 }
 
+// Fine (sub-cell) coordinates of a particle of a dense cell, 21 bits per axis,
+// recorded once by k_subsort and read by the link kernels.
+__device__ __forceinline__ unsigned long long pack_fine(int fx, int fy, int fz) {
+  return (unsigned long long)fx | ((unsigned long long)fy << 21) | ((unsigned long long)fz << 42);
+}
+__device__ __forceinline__ int fine_x(unsigned long long f) { return (int)(f & 0x1FFFFFull); }
+__device__ __forceinline__ int fine_y(unsigned long long f) { return (int)((f >> 21) & 0x1FFFFFull); }
+__device__ __forceinline__ int fine_z(unsigned long long f) { return (int)((f >> 42) & 0x1FFFFFull); }
+
 __device__ __forceinline__ uint32_t cell_id(const VolDev &V, int ix, int iy, int iz) {
   return V.cbeg + ((uint32_t)iz * (uint32_t)V.ny + (uint32_t)iy) * (uint32_t)V.nx +
          (uint32_t)ix;
@@ -312,8 +321,37 @@ __global__ __launch_bounds__(256) void k_keys(const uint8_t *__restrict__ base, 
     const int iz = ok ? cell1((double)z, V.rmin_buff[2], V.lcell[2], V.nz, &u) : 0;
     const uint32_t c = cell_id(V, ix, iy, iz);
     key[p] = c;
-    const uint32_t old = atomicAdd(cnt + c, 1u);
-    if (old + 1 == T && V.m[0] > 0) dense[atomicAdd(ndense, 1u)] = c;
+    atomicAdd(cnt + c, 1u);  // result unused: a non-returning atomic
+  }
+}
+
+// Dense-cell list from the cell counts (one coalesced pass over the cells;
+// wave-aggregated appends).
+__global__ __launch_bounds__(256) void k_dense_list(uint32_t C, const uint32_t *__restrict__ cnt,
+                                                    const VolDev *__restrict__ vols, int nvol,
+                                                    uint32_t T, uint32_t *__restrict__ dense,
+                                                    uint32_t *__restrict__ ndense) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t c0 = (blockIdx.x * blockDim.x) & ~63u; c0 < C; c0 += gridDim.x * blockDim.x) {
+    const uint32_t c = c0 + lane + (threadIdx.x & ~63u);
+    bool is = false;
+    if (c < C && cnt[c] >= T) {
+      // volume of cell c (cells are laid out volume after volume)
+      int lo = 0, hi = nvol - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (vols[mid].cbeg <= c) lo = mid;
+        else hi = mid - 1;
+      }
+      is = vols[lo].m[0] > 0;
+    }
+    const unsigned long long b = __ballot(is);
+    if (b) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(ndense, (uint32_t)__popcll(b));
+      base = __shfl(base, 0, 64);
+      if (is) dense[base + __popcll(b & ((1ull << lane) - 1))] = c;
+    }
   }
 }
 
@@ -459,8 +497,13 @@ __global__ __launch_bounds__(256) void k_subsort(const uint32_t *__restrict__ de
                                                  float4 *__restrict__ pos4,
                                                  float4 *__restrict__ tmp4,
                                                  uint32_t *__restrict__ par,
-                                                 uint32_t *__restrict__ run_end) {
+                                                 uint32_t *__restrict__ run_end,
+                                                 unsigned long long *__restrict__ fine,
+                                                 uint32_t *__restrict__ rtoff,
+                                                 uint32_t *__restrict__ rtab,
+                                                 uint32_t *__restrict__ rtcount) {
   __shared__ uint32_t bstart[kMaxBins];
+  __shared__ uint32_t rbase;
   __shared__ uint32_t bcur[kMaxBins];
   __shared__ uint32_t ws[8];
   const uint32_t nd = *ndense;
@@ -507,12 +550,46 @@ __global__ __launch_bounds__(256) void k_subsort(const uint32_t *__restrict__ de
       tmp4[s + slot] = p;
     }
     __syncthreads();
+    // run table of the cell: [count, heads in sub-cell order], bump-allocated;
+    // rtoff at the cell's first particle points to it
+    {
+      const int b0 = threadIdx.x * 16;
+      uint32_t loc = 0;
+      for (int q = 0; q < 16; q++)
+        if (b0 + q < nbins && bcur[b0 + q] != bstart[b0 + q]) loc++;
+      uint32_t inc = loc;
+      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+      }
+      if (lane == 63) ws[w] = inc;
+      __syncthreads();
+      uint32_t off = 0, tot = 0;
+      for (int q = 0; q < 4; q++) {
+        off += (q < w) ? ws[q] : 0u;
+        tot += ws[q];
+      }
+      if (threadIdx.x == 0) {
+        rbase = atomicAdd(rtcount, tot + 1);
+        rtab[rbase] = tot;
+        rtoff[s] = rbase;
+      }
+      __syncthreads();
+      uint32_t r = rbase + 1 + off + inc - loc;
+      for (int q = 0; q < 16; q++)
+        if (b0 + q < nbins && bcur[b0 + q] != bstart[b0 + q]) rtab[r++] = s + bstart[b0 + q];
+    }
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
       const float4 p = tmp4[s + i];
-      const int b = sub_bin(V, p);
+      const CellPos cp = cell_pos(V, p.x, p.y, p.z);
+      const int b = ((cp.fz - cp.iz * V.m[2]) * V.m[1] + (cp.fy - cp.iy * V.m[1])) * V.m[0] +
+                    (cp.fx - cp.ix * V.m[0]);
       pos4[s + i] = p;
       par[s + i] = s + bstart[b];
       run_end[s + i] = s + bcur[b];
+      fine[s + i] = pack_fine(cp.fx, cp.fy, cp.fz);
     }
     __syncthreads();
   }
@@ -524,6 +601,7 @@ __global__ __launch_bounds__(256) void k_subsort(const uint32_t *__restrict__ de
 __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *__restrict__ pos4,
                                                      const Grid G,
                                                      const uint32_t *__restrict__ run_end,
+                                                     const unsigned long long *__restrict__ fine,
                                                      uint32_t *__restrict__ par,
                                                      const VolDev *__restrict__ vols,
                                                      const uint32_t *__restrict__ vbeg,
@@ -603,11 +681,10 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
           continue;
         }
         // a clique run of a dense cell: skip it when out of reach or joined
-        const float4 pj = pos4[j];
-        const CellPos cj = cell_pos(V, pj.x, pj.y, pj.z);
-        const int gx = max(abs(cj.fx - cp.fx) - 1, 0);
-        const int gy = max(abs(cj.fy - cp.fy) - 1, 0);
-        const int gz = max(abs(cj.fz - cp.fz) - 1, 0);
+        const unsigned long long fj = fine[j];
+        const int gx = max(abs(fine_x(fj) - cp.fx) - 1, 0);
+        const int gy = max(abs(fine_y(fj) - cp.fy) - 1, 0);
+        const int gz = max(abs(fine_z(fj) - cp.fz) - 1, 0);
         const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
                           (double)(gz * gz) * V.s2[2];
         if (!(g2 > R.prune2) && uf_find(par, rx) != uf_find(par, j)) {
@@ -635,7 +712,7 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
 // (reach <= 2 when m >= 5, reach <= 4 and m <= 4 otherwise) and fewer than
 // T <= kMaxT particles for a sparse one.
 constexpr int kMaxRuns = 1728;
-constexpr int kMaxRunsB = 512;
+constexpr int kMaxRunsB = 512;  // neighbour particles scanned per chunk
 constexpr uint32_t kMaxT = 512;
 
 __device__ __forceinline__ uint32_t pack_rel(int a, int b, int c) {
@@ -650,19 +727,28 @@ __device__ __forceinline__ int rel_z(uint32_t r) { return (int)(int8_t)((r >> 16
 // passes the face filter: for each axis with window w[d] = (lo,hi), only
 // runs with lo <= rel <= hi.
 __device__ int collect_runs(const float4 *__restrict__ pos4,
-                            const uint32_t *__restrict__ run_end, uint32_t s, uint32_t e,
-                            const VolDev &V, int ox, int oy, int oz, int wlo0, int whi0,
-                            int wlo1, int whi1, int wlo2, int whi2, uint32_t *head,
-                            uint32_t *end, uint32_t *rel, uint32_t *cnt, uint32_t cap) {
+                            const uint32_t *__restrict__ run_end,
+                            const unsigned long long *__restrict__ fine, bool dense_cell,
+                            uint32_t s, uint32_t e, const VolDev &V, int ox, int oy, int oz,
+                            int wlo0, int whi0, int wlo1, int whi1, int wlo2, int whi2,
+                            uint32_t *head, uint32_t *end, uint32_t *rel, uint32_t *cnt,
+                            uint32_t cap) {
   if (threadIdx.x == 0) *cnt = 0;
   __syncthreads();
   for (uint32_t j = s + threadIdx.x; j < e; j += blockDim.x) {
     const uint32_t re = run_end[j];
     const bool isstart = (j == s) || (run_end[j - 1] != re);
     if (!isstart) continue;
-    const float4 pj = pos4[j];
-    const CellPos c = cell_pos(V, pj.x, pj.y, pj.z);
-    const int rx = c.fx - ox, ry = c.fy - oy, rz = c.fz - oz;
+    int fx, fy, fz;
+    if (dense_cell) {  // sub-sorted: fine coordinates recorded by k_subsort
+      const unsigned long long f = fine[j];
+      fx = fine_x(f), fy = fine_y(f), fz = fine_z(f);
+    } else {
+      const float4 pj = pos4[j];
+      const CellPos c = cell_pos(V, pj.x, pj.y, pj.z);
+      fx = c.fx, fy = c.fy, fz = c.fz;
+    }
+    const int rx = fx - ox, ry = fy - oy, rz = fz - oz;
     if (rx < wlo0 || rx > whi0 || ry < wlo1 || ry > whi1 || rz < wlo2 || rz > whi2) continue;
     const uint32_t k = atomicAdd(cnt, 1u);
     if (k < cap) {
@@ -736,53 +822,94 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
                                                     const float4 *__restrict__ pos4,
                                                     const Grid G,
                                                     const uint32_t *__restrict__ run_end,
+                                                    const unsigned long long *__restrict__ fine,
+                                                    const uint32_t *__restrict__ rtoff,
+                                                    const uint32_t *__restrict__ rtab,
+                                                    uint32_t T,
                                                     uint32_t *__restrict__ par,
                                                     const VolDev *__restrict__ vols,
                                                     const uint32_t *__restrict__ vbeg,
                                                     int nvol,
                                                     unsigned long long *__restrict__ ctr,
                                                     uint32_t *__restrict__ overflow) {
-  __shared__ uint32_t ahead[kMaxRuns], aend[kMaxRuns], arel[kMaxRuns];
-  __shared__ uint32_t bhead[kMaxRunsB], bend[kMaxRunsB], brel[kMaxRunsB];
+  __shared__ uint32_t ahead[kMaxRuns], arel[kMaxRuns], agr[kMaxRuns];
+  __shared__ uint32_t bhead[kMaxRunsB], brel[kMaxRunsB], bgr[kMaxRunsB];
   __shared__ uint32_t lp[kMaxRuns + kMaxRunsB];
-  __shared__ uint16_t amap[kMaxRuns];  // sub-cell -> run (A,A); filtered A runs (A,B)
-  uint16_t *const aidx = amap;
-  __shared__ uint32_t cnt, acnt;
+  __shared__ uint16_t amap[kMaxRuns];  // sub-cell of A -> run slot
+  __shared__ uint32_t nbS[13], nbOff[14], nbTab[13];
+  __shared__ int nbLo[13][3], nbHi[13][3];
+  __shared__ uint8_t nbDense[13];
+  __shared__ uint32_t cnt;
   uint32_t tests = 0, rechecks = 0;
   const uint32_t nd = *ndense;
   for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
     const uint32_t c = dense[d];
     const uint32_t sA = G.start[c], eA = G.start[c + 1];
     const VolDev &V = vols[vol_of(vbeg, nvol, sA)];
-    const float4 p0 = pos4[sA];
-    const CellPos c0 = cell_pos(V, p0.x, p0.y, p0.z);
-    const int ix = c0.ix, iy = c0.iy, iz = c0.iz;
-    const Rule R = make_rule(V, ix, iy, iz);
+    const unsigned long long f0 = fine[sA];
     const int mm[3] = {V.m[0], V.m[1], V.m[2]};
+    const int ix = fine_x(f0) / mm[0], iy = fine_y(f0) / mm[1], iz = fine_z(f0) / mm[2];
+    const Rule R = make_rule(V, ix, iy, iz);
     const int ox = ix * mm[0], oy = iy * mm[1], oz = iz * mm[2];
-    const int nbins = mm[0] * mm[1] * mm[2];
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = 0xFFFF;  // nbins <= 1728
-    // the home cell's runs (one per occupied sub-cell), once
-    const int na = collect_runs(pos4, run_end, sA, eA, V, ox, oy, oz, -1000, 1000, -1000, 1000, -1000,
-                                1000, ahead, aend, arel, &cnt, kMaxRuns);
-    if (na > kMaxRuns) {
-      if (threadIdx.x == 0) atomicOr(overflow, 1u);
-      continue;
-    }
-    for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
-      const uint32_t r = arel[ia];
-      amap[(rel_z(r) * mm[1] + rel_y(r)) * mm[0] + rel_x(r)] = (uint16_t)ia;
-      lp[ia] = ia;
-    }
-    __syncthreads();
+    const int nbins = mm[0] * mm[1] * mm[2];  // <= 1728
     int reach[3];
     for (int a = 0; a < 3; a++)  // largest sub-cell offset whose gap can be within reach
       reach[a] = min((int)floor(sqrt(R.prune2 / V.s2[a])) + 1, mm[a]);
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = 0xFFFF;
+    // neighbour table: the 13 forward cells of src_v2/FOF.c:122-153, their
+    // particle ranges and the window of their runs that can reach A
+    if (threadIdx.x < 13) {
+      const int k = threadIdx.x;
+      int dd[3] = {0, 0, 0};
+      if (k == 0) dd[0] = 1;
+      else if (k < 4) dd[1] = 1, dd[0] = k - 2;
+      else dd[2] = 1, dd[1] = (k - 4) / 3 - 1, dd[0] = (k - 4) % 3 - 1;
+      const int bx = ix + dd[0], by = iy + dd[1], bz = iz + dd[2];
+      uint32_t sB = 0, eB = 0;
+      if (!(bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz >= V.nz)) {
+        const uint32_t cb = cell_index(G, V, bx, by, bz);
+        if (cb != kNone) sB = G.start[cb], eB = G.start[cb + 1];
+      }
+      // items of a neighbour: its runs (dense, from the run table) or particles
+      const bool dn = (eB - sB >= T);
+      nbS[k] = sB;
+      nbDense[k] = dn ? 1 : 0;
+      nbTab[k] = dn ? rtoff[sB] : 0u;
+      nbOff[k + 1] = dn ? rtab[nbTab[k]] : eB - sB;
+      for (int a = 0; a < 3; a++) {
+        int lo = -1000, hi = 1000;
+        if (dd[a] == 1) lo = mm[a], hi = mm[a] - 1 + reach[a];
+        else if (dd[a] == -1) lo = -reach[a], hi = -1;
+        nbLo[k][a] = lo, nbHi[k][a] = hi;
+      }
+    }
+    // the home cell's runs (one per occupied sub-cell)
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      nbOff[0] = 0;
+      for (int k = 0; k < 13; k++) nbOff[k + 1] += nbOff[k];
+    }
+    const uint32_t atab = rtoff[sA];
+    const uint32_t nruns = rtab[atab];
+    for (uint32_t t = threadIdx.x; t < nruns; t += blockDim.x) {
+      const uint32_t j = rtab[atab + 1 + t];
+      const unsigned long long f = fine[j];
+      const int rx = fine_x(f) - ox, ry = fine_y(f) - oy, rz = fine_z(f) - oz;
+      const uint32_t k = atomicAdd(&cnt, 1u);  // <= nbins <= kMaxRuns
+      ahead[k] = j;
+      arel[k] = pack_rel(rx, ry, rz);
+      agr[k] = uf_find(par, j);  // snapshot of the global root
+      lp[k] = k;
+      amap[(rz * mm[1] + ry) * mm[0] + rx] = (uint16_t)k;
+    }
+    __syncthreads();
+    const int na = (int)cnt;
     // (A, A): forward sub-cell offsets within reach, through the sub-cell map
     for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
       const uint32_t ra = arel[ia];
       const int ax = rel_x(ra), ay = rel_y(ra), az = rel_z(ra);
-      const uint32_t ha = ahead[ia], ea = aend[ia];
+      const uint32_t ha = ahead[ia], ea = run_end[ha];
       for (int dz = 0; dz <= reach[2]; dz++) {
         const int bz = az + dz;
         if (bz >= mm[2]) break;
@@ -803,88 +930,88 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
             const uint32_t ib = amap[(bz * mm[1] + by) * mm[0] + bxx];
             if (ib == 0xFFFF) continue;
             if (lfind(lp, ia) == lfind(lp, ib)) continue;
-            // joined elsewhere (other workgroups, sparse-cell threads)?
-            if (uf_find(par, ha) == uf_find(par, ahead[ib]) ||
-                runs_link(pos4, ha, ea, ahead[ib], aend[ib], R, tests, rechecks))
+            const uint32_t hb = ahead[ib];
+            // joined elsewhere (snapshot of the global roots), or linked here
+            if (agr[ia] == agr[ib] || runs_link(pos4, ha, ea, hb, run_end[hb], R, tests, rechecks))
               lunion(lp, ia, ib);
           }
         }
       }
     }
     __syncthreads();
-    // publish the home cell's merges before the neighbour passes
+    // publish the home cell's merges before the neighbour pass
     for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
       const uint32_t r = lfind(lp, ia);
       if (r != (uint32_t)ia) uf_union(par, ahead[ia], ahead[r]);
     }
-    __syncthreads();
-    // (A, B) for the 13 forward cells of src_v2/FOF.c:122-153
-    for (int q = 1; q < 14; q++) {
-      int dd[3] = {0, 0, 0};
-      const int k = q - 1;  // 0..12
-      if (k == 0) dd[0] = 1;
-      else if (k < 4) dd[1] = 1, dd[0] = k - 2;
-      else dd[2] = 1, dd[1] = (k - 4) / 3 - 1, dd[0] = (k - 4) % 3 - 1;
-      const int bx = ix + dd[0], by = iy + dd[1], bz = iz + dd[2];
-      if (bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz >= V.nz) continue;
-      const uint32_t cb = cell_index(G, V, bx, by, bz);
-      if (cb == kNone) continue;
-      const uint32_t sB = G.start[cb], eB = G.start[cb + 1];
-      if (sB == eB) continue;
-      // only runs within reach of the shared face / edge / corner
-      int alo[3], ahi[3], blo[3], bhi[3];
-      for (int a = 0; a < 3; a++) {
-        alo[a] = -1000, ahi[a] = 1000, blo[a] = -1000, bhi[a] = 1000;
-        if (dd[a] == 1) alo[a] = mm[a] - reach[a], blo[a] = mm[a], bhi[a] = mm[a] - 1 + reach[a];
-        else if (dd[a] == -1) ahi[a] = reach[a] - 1, blo[a] = -reach[a], bhi[a] = -1;
-      }
-      const int nb = collect_runs(pos4, run_end, sB, eB, V, ox, oy, oz, blo[0], bhi[0], blo[1],
-                                  bhi[1], blo[2], bhi[2], bhead, bend, brel, &cnt, kMaxRunsB);
-      if (nb > kMaxRunsB) {
-        if (threadIdx.x == 0) atomicOr(overflow, 1u);
-        continue;
-      }
-      if (nb == 0) continue;
-      if (threadIdx.x == 0) acnt = 0;
-      for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) lp[na + ib] = na + ib;
+    // (A, B): the neighbours' particles in chunks of kMaxRunsB
+    const uint32_t ntot = nbOff[13];
+    for (uint32_t c0 = 0; c0 < ntot; c0 += kMaxRunsB) {
+      const uint32_t c1 = min(ntot, c0 + (uint32_t)kMaxRunsB);
+      if (threadIdx.x == 0) cnt = 0;
       __syncthreads();
-      for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
-        const uint32_t r = arel[ia];
-        const int rx = rel_x(r), ry = rel_y(r), rz = rel_z(r);
-        if (rx >= alo[0] && rx <= ahi[0] && ry >= alo[1] && ry <= ahi[1] && rz >= alo[2] &&
-            rz <= ahi[2])
-          aidx[atomicAdd(&acnt, 1u)] = (uint16_t)ia;
+      for (uint32_t g = c0 + threadIdx.x; g < c1; g += blockDim.x) {
+        int q = 0;
+        while (nbOff[q + 1] <= g) q++;
+        const uint32_t item = g - nbOff[q];
+        const uint32_t j = nbDense[q] ? rtab[nbTab[q] + 1 + item] : nbS[q] + item;
+        int fx, fy, fz;
+        if (nbDense[q]) {
+          const unsigned long long f = fine[j];
+          fx = fine_x(f), fy = fine_y(f), fz = fine_z(f);
+        } else {
+          const float4 pj = pos4[j];
+          const CellPos cj = cell_pos(V, pj.x, pj.y, pj.z);
+          fx = cj.fx, fy = cj.fy, fz = cj.fz;
+        }
+        const int rx = fx - ox, ry = fy - oy, rz = fz - oz;
+        if (rx < nbLo[q][0] || rx > nbHi[q][0] || ry < nbLo[q][1] || ry > nbHi[q][1] ||
+            rz < nbLo[q][2] || rz > nbHi[q][2])
+          continue;
+        const uint32_t k = atomicAdd(&cnt, 1u);  // < kMaxRunsB: one run per particle at most
+        bhead[k] = j;
+        brel[k] = pack_rel(rx, ry, rz);
+        bgr[k] = uf_find(par, j);
+        lp[na + k] = na + k;
       }
       __syncthreads();
-      const int n1 = (int)acnt;
-      // pairs (filtered A run, B run), B-major so threads share A's roots
-      const int npairs = n1 * nb;
-      for (int t = threadIdx.x; t < npairs; t += blockDim.x) {
-        const int ib = t / n1, ia = aidx[t - ib * n1];
-        const uint32_t ra = arel[ia], rb = brel[ib];
-        const int gx = max(abs(rel_x(rb) - rel_x(ra)) - 1, 0);
-        const int gy = max(abs(rel_y(rb) - rel_y(ra)) - 1, 0);
-        const int gz = max(abs(rel_z(rb) - rel_z(ra)) - 1, 0);
-        const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
-                          (double)(gz * gz) * V.s2[2];
-        if (g2 > R.prune2) continue;
-        if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
-        if (uf_find(par, ahead[ia]) == uf_find(par, bhead[ib]) ||
-            runs_link(pos4, ahead[ia], aend[ia], bhead[ib], bend[ib], R, tests, rechecks))
-          lunion(lp, ia, na + ib);
-      }
-      __syncthreads();
-      // commit the neighbour's runs: each joins its LDS root's run globally
+      const int nb = (int)cnt;
+      // each neighbour run against the home sub-cells within reach
       for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
-        const uint32_t r = lfind(lp, na + ib);
-        if (r != (uint32_t)(na + ib)) {
-          const uint32_t hr = (r < (uint32_t)na) ? ahead[r] : bhead[r - na];
-          uf_union(par, bhead[ib], hr);
+        const uint32_t rb = brel[ib];
+        const int bx = rel_x(rb), by = rel_y(rb), bz = rel_z(rb);
+        const uint32_t hb = bhead[ib], eb = run_end[hb];
+        const int z0 = max(bz - reach[2], 0), z1 = min(bz + reach[2], mm[2] - 1);
+        const int y0 = max(by - reach[1], 0), y1 = min(by + reach[1], mm[1] - 1);
+        const int x0 = max(bx - reach[0], 0), x1 = min(bx + reach[0], mm[0] - 1);
+        for (int az = z0; az <= z1; az++) {
+          const int gz = max(abs(az - bz) - 1, 0);
+          for (int ay = y0; ay <= y1; ay++) {
+            const int gy = max(abs(ay - by) - 1, 0);
+            for (int ax = x0; ax <= x1; ax++) {
+              const int gx = max(abs(ax - bx) - 1, 0);
+              const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
+                                (double)(gz * gz) * V.s2[2];
+              if (g2 > R.prune2) continue;
+              const uint32_t ia = amap[(az * mm[1] + ay) * mm[0] + ax];
+              if (ia == 0xFFFF) continue;
+              if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
+              const uint32_t ha = ahead[ia];
+              if (agr[ia] == bgr[ib] || runs_link(pos4, ha, run_end[ha], hb, eb, R, tests, rechecks))
+                lunion(lp, ia, na + ib);
+            }
+          }
         }
       }
       __syncthreads();
+      // commit: a neighbour run joined in LDS joins its (home-cell) root's run
+      for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
+        const uint32_t r = lfind(lp, na + ib);
+        if (r != (uint32_t)(na + ib)) uf_union(par, bhead[ib], ahead[r]);
+      }
+      __syncthreads();
     }
-    // commit the home cell's runs
+    // commit the home cell's runs (merges made through neighbours)
     for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
       const uint32_t r = lfind(lp, ia);
       if (r != (uint32_t)ia) uf_union(par, ahead[ia], ahead[r]);
@@ -1179,6 +1306,9 @@ int prepare_volume(const fof_volume_v1 &in, double band, HostVol &hv, std::vecto
       if (q > 12) ok = false;
       m[k] = (int)q;
     }
+    const int nn[3] = {V.nx, V.ny, V.nz};
+    for (int k = 0; k < 3; k++)
+      if ((double)nn[k] * m[k] >= 2097152.0) ok = false;  // 21-bit fine coordinates
     if (ok)
       for (int k = 0; k < 3; k++) {
         V.m[k] = m[k];
@@ -1291,6 +1421,9 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
           if (q > 12) ok = false;
           m[k] = (int)q;
         }
+        const int nn[3] = {V.nx, V.ny, V.nz};
+        for (int k = 0; k < 3; k++)
+          if ((double)nn[k] * m[k] >= 2097152.0) ok = false;  // 21-bit fine coordinates
         if (ok)
           for (int k = 0; k < 3; k++) {
             V.m[k] = m[k];
@@ -1327,6 +1460,8 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if ((rc = ensure(W.start, (Csz + 2) * 4 + 64))) return rc;
   if ((rc = ensure(W.par, N * 4 + 64))) return rc;
   if ((rc = ensure(W.runend, N * 4 + 64))) return rc;
+  if ((rc = ensure(W.fine, N * 8 + 64))) return rc;
+  if ((rc = ensure(W.rtab, (N + ndense_cap + 64) * 4))) return rc;
   if ((rc = ensure(W.gsize, N * 4 + 64))) return rc;
   if ((rc = ensure(W.label, N * 4 + 64))) return rc;
   const uint64_t q4 = (N + 63) / 64 * 64;  // u32 entries per sort array
@@ -1387,6 +1522,9 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   uint32_t *start = static_cast<uint32_t *>(W.start.ptr);
   uint32_t *par = static_cast<uint32_t *>(W.par.ptr);
   uint32_t *runend = static_cast<uint32_t *>(W.runend.ptr);
+  unsigned long long *fine = static_cast<unsigned long long *>(W.fine.ptr);
+  uint32_t *rtab = static_cast<uint32_t *>(W.rtab.ptr);
+  uint32_t *rtoff = key;  // free between the cell sort and k_label (head sizes)
   uint32_t *gsize = static_cast<uint32_t *>(W.gsize.ptr);
   uint32_t *label = static_cast<uint32_t *>(W.label.ptr);
   // four u32 arrays; viewed as float4[N] they are the sub-cell sort buffer
@@ -1439,6 +1577,8 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     if ((rc = tb(0))) return rc;
     k_keys<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, key, cnt, T,
                                dense, small + 1, small + 0);
+    FOF_HIP(hipGetLastError());
+    k_dense_list<<<grid_for(C), 256, 0, st>>>((uint32_t)C, cnt, dV, (int)nvol, T, dense, small + 1);
     FOF_HIP(hipGetLastError());
     if ((rc = te(0))) return rc;
     FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
@@ -1507,18 +1647,20 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if (ndense) {
     if ((rc = tb(4))) return rc;
     k_subsort<<<gD, 256, 0, st>>>(dense, small + 1, start, dV, dVB, (int)nvol, pos4,
-                                  reinterpret_cast<float4 *>(sk0), par, runend);
+                                  reinterpret_cast<float4 *>(sk0), par, runend, fine, rtoff, rtab,
+                                  small + 6);
     FOF_HIP(hipGetLastError());
     if ((rc = te(4))) return rc;
   }
   // 5-6: linking
   if ((rc = tb(5))) return rc;
-  k_link_sparse<<<gN, 256, 0, st>>>(NN, pos4, G, runend, par, dV, dVB, (int)nvol, T, ctr);
+  k_link_sparse<<<gN, 256, 0, st>>>(NN, pos4, G, runend, fine, par, dV, dVB, (int)nvol, T, ctr);
   FOF_HIP(hipGetLastError());
   if ((rc = te(5))) return rc;
   if (ndense) {
     if ((rc = tb(6))) return rc;
-    k_link_dense<<<gD, 256, 0, st>>>(dense, small + 1, pos4, G, runend, par, dV, dVB,
+    k_link_dense<<<gD, 256, 0, st>>>(dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par,
+                                     dV, dVB,
                                      (int)nvol, ctr, small + 4);
     FOF_HIP(hipGetLastError());
     if ((rc = te(6))) return rc;
@@ -1639,7 +1781,7 @@ extern "C" void fof_engine_free(fof_ctx *ctx) {
                     &S.w.gsize, &S.w.label, &S.w.sortbuf,
                     &S.w.dense, &S.w.tmp, &S.w.vols, &S.w.vbeg, &S.w.small, &S.w.input,
                     &S.w.out_next, &S.w.out_label, &S.w.out_halo, &S.w.hhead, &S.w.hoff,
-                    &S.w.hash};
+                    &S.w.hash, &S.w.fine, &S.w.rtab};
   for (Scratch *s : all) release(*s);
   if (S.events)
     for (auto &e : S.ev) (void)hipEventDestroy(e);

@@ -124,7 +124,7 @@ class Stats(C.Structure):
         ("nhalos", C.c_uint64),
         ("nhalos_owned", C.c_uint64),
         ("sparse", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("dense_threshold", C.c_uint32),
         ("phase_ms", C.c_double * NPHASE),
         ("link_kernel_ms", C.c_double),
     ]

@@ -707,13 +707,14 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
 // 6: dense cells -- one workgroup per cell, run pairs with early exit.
 // ---------------------------------------------------------------------------
 // Run-table capacities.  The home cell has one run per occupied sub-cell,
-// at most 12^3.  A neighbour contributes only runs within reach of the
-// shared face/edge/corner: at most reach*m*m <= 288 for a dense neighbour
-// (reach <= 2 when m >= 5, reach <= 4 and m <= 4 otherwise) and fewer than
-// T <= kMaxT particles for a sparse one.
+// at most 12^3; neighbour runs (dense neighbours) or particles (sparse
+// neighbours) are taken kMaxRunsB at a time.
 constexpr int kMaxRuns = 1728;
-constexpr int kMaxRunsB = 512;  // neighbour particles scanned per chunk
+constexpr int kMaxRunsB = 512;  // neighbour items per chunk
 constexpr uint32_t kMaxT = 512;
+// Default dense threshold: the sweep over clumpy 1024^3 boxes (b = 0.2,
+// about one particle per cell) is flat within 5% from 64 to 256.
+constexpr uint32_t kDefaultT = 96;
 
 __device__ __forceinline__ uint32_t pack_rel(int a, int b, int c) {
   return ((uint32_t)(a & 255)) | ((uint32_t)(b & 255) << 8) | ((uint32_t)(c & 255) << 16);
@@ -721,47 +722,6 @@ __device__ __forceinline__ uint32_t pack_rel(int a, int b, int c) {
 __device__ __forceinline__ int rel_x(uint32_t r) { return (int)(int8_t)(r & 255); }
 __device__ __forceinline__ int rel_y(uint32_t r) { return (int)(int8_t)((r >> 8) & 255); }
 __device__ __forceinline__ int rel_z(uint32_t r) { return (int)(int8_t)((r >> 16) & 255); }
-
-// Collect the runs of sorted range [s,e) into LDS (head, end, fine coordinate
-// relative to the home cell's corner).  Keeps runs whose relative coordinate
-// passes the face filter: for each axis with window w[d] = (lo,hi), only
-// runs with lo <= rel <= hi.
-__device__ int collect_runs(const float4 *__restrict__ pos4,
-                            const uint32_t *__restrict__ run_end,
-                            const unsigned long long *__restrict__ fine, bool dense_cell,
-                            uint32_t s, uint32_t e, const VolDev &V, int ox, int oy, int oz,
-                            int wlo0, int whi0, int wlo1, int whi1, int wlo2, int whi2,
-                            uint32_t *head, uint32_t *end, uint32_t *rel, uint32_t *cnt,
-                            uint32_t cap) {
-  if (threadIdx.x == 0) *cnt = 0;
-  __syncthreads();
-  for (uint32_t j = s + threadIdx.x; j < e; j += blockDim.x) {
-    const uint32_t re = run_end[j];
-    const bool isstart = (j == s) || (run_end[j - 1] != re);
-    if (!isstart) continue;
-    int fx, fy, fz;
-    if (dense_cell) {  // sub-sorted: fine coordinates recorded by k_subsort
-      const unsigned long long f = fine[j];
-      fx = fine_x(f), fy = fine_y(f), fz = fine_z(f);
-    } else {
-      const float4 pj = pos4[j];
-      const CellPos c = cell_pos(V, pj.x, pj.y, pj.z);
-      fx = c.fx, fy = c.fy, fz = c.fz;
-    }
-    const int rx = fx - ox, ry = fy - oy, rz = fz - oz;
-    if (rx < wlo0 || rx > whi0 || ry < wlo1 || ry > whi1 || rz < wlo2 || rz > whi2) continue;
-    const uint32_t k = atomicAdd(cnt, 1u);
-    if (k < cap) {
-      head[k] = j;
-      end[k] = re;
-      rel[k] = pack_rel(rx, ry, rz);
-    }
-  }
-  __syncthreads();
-  const uint32_t n = *cnt;
-  __syncthreads();
-  return (int)n;
-}
 
 // Union-find over run slots in LDS (A's runs 0..na-1, the neighbour's runs
 // after them).  Hooking larger-under-smaller keeps A's roots inside A, so the
@@ -905,35 +865,54 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
     }
     __syncthreads();
     const int na = (int)cnt;
-    // (A, A): forward sub-cell offsets within reach, through the sub-cell map
-    for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
-      const uint32_t ra = arel[ia];
-      const int ax = rel_x(ra), ay = rel_y(ra), az = rel_z(ra);
-      const uint32_t ha = ahead[ia], ea = run_end[ha];
-      for (int dz = 0; dz <= reach[2]; dz++) {
-        const int bz = az + dz;
-        if (bz >= mm[2]) break;
-        const int gz = max(dz - 1, 0);
-        for (int dy = -reach[1]; dy <= reach[1]; dy++) {
-          if (dz == 0 && dy < 0) continue;
-          const int by = ay + dy;
-          if (by < 0 || by >= mm[1]) continue;
-          const int gy = max(abs(dy) - 1, 0);
-          for (int dx = -reach[0]; dx <= reach[0]; dx++) {
-            if (dz == 0 && dy == 0 && dx <= 0) continue;
-            const int bxx = ax + dx;
-            if (bxx < 0 || bxx >= mm[0]) continue;
-            const int gx = max(abs(dx) - 1, 0);
-            const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
-                              (double)(gz * gz) * V.s2[2];
-            if (g2 > R.prune2) continue;
-            const uint32_t ib = amap[(bz * mm[1] + by) * mm[0] + bxx];
-            if (ib == 0xFFFF) continue;
-            if (lfind(lp, ia) == lfind(lp, ib)) continue;
-            const uint32_t hb = ahead[ib];
-            // joined elsewhere (snapshot of the global roots), or linked here
-            if (agr[ia] == agr[ib] || runs_link(pos4, ha, ea, hb, run_end[hb], R, tests, rechecks))
-              lunion(lp, ia, ib);
+    // Sub-cell gap pruning in FP32: the prune radius carries a 1e-4 margin,
+    // far above the FP32 rounding of the gap.
+    const float gs0 = (float)V.s2[0], gs1 = (float)V.s2[1], gs2 = (float)V.s2[2];
+    const float gp2 = (float)R.prune2;
+    auto gap_ok = [&](int dx, int dy, int dz) {
+      const int gx = max(abs(dx) - 1, 0), gy = max(abs(dy) - 1, 0), gz = max(abs(dz) - 1, 0);
+      return (float)(gx * gx) * gs0 + (float)(gy * gy) * gs1 + (float)(gz * gz) * gs2 <= gp2;
+    };
+    // (A, A): every run pair, or forward sub-cell offsets through the map
+    // when the cell has more runs than the offset window
+    const int wAA = (reach[2] + 1) * (2 * reach[1] + 1) * (2 * reach[0] + 1);
+    if (na <= wAA) {
+      const uint32_t npair = (uint32_t)na * (uint32_t)na;
+      for (uint32_t p = threadIdx.x; p < npair; p += blockDim.x) {
+        const uint32_t ia = p / (uint32_t)na, ib = p - ia * (uint32_t)na;
+        if (ib <= ia) continue;
+        const uint32_t ra = arel[ia], rb = arel[ib];
+        if (!gap_ok(rel_x(rb) - rel_x(ra), rel_y(rb) - rel_y(ra), rel_z(rb) - rel_z(ra))) continue;
+        if (lfind(lp, ia) == lfind(lp, ib)) continue;
+        const uint32_t ha = ahead[ia], hb = ahead[ib];
+        // joined elsewhere (snapshot of the global roots), or linked here
+        if (agr[ia] == agr[ib] || runs_link(pos4, ha, run_end[ha], hb, run_end[hb], R, tests, rechecks))
+          lunion(lp, ia, ib);
+      }
+    } else {
+      for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
+        const uint32_t ra = arel[ia];
+        const int ax = rel_x(ra), ay = rel_y(ra), az = rel_z(ra);
+        const uint32_t ha = ahead[ia], ea = run_end[ha];
+        for (int dz = 0; dz <= reach[2]; dz++) {
+          const int bz = az + dz;
+          if (bz >= mm[2]) break;
+          for (int dy = -reach[1]; dy <= reach[1]; dy++) {
+            if (dz == 0 && dy < 0) continue;
+            const int by = ay + dy;
+            if (by < 0 || by >= mm[1]) continue;
+            for (int dx = -reach[0]; dx <= reach[0]; dx++) {
+              if (dz == 0 && dy == 0 && dx <= 0) continue;
+              const int bxx = ax + dx;
+              if (bxx < 0 || bxx >= mm[0]) continue;
+              if (!gap_ok(dx, dy, dz)) continue;
+              const uint32_t ib = amap[(bz * mm[1] + by) * mm[0] + bxx];
+              if (ib == 0xFFFF) continue;
+              if (lfind(lp, ia) == lfind(lp, ib)) continue;
+              const uint32_t hb = ahead[ib];
+              if (agr[ia] == agr[ib] || runs_link(pos4, ha, ea, hb, run_end[hb], R, tests, rechecks))
+                lunion(lp, ia, ib);
+            }
           }
         }
       }
@@ -976,31 +955,43 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
       }
       __syncthreads();
       const int nb = (int)cnt;
-      // each neighbour run against the home sub-cells within reach
-      for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
-        const uint32_t rb = brel[ib];
-        const int bx = rel_x(rb), by = rel_y(rb), bz = rel_z(rb);
-        const uint32_t hb = bhead[ib], eb = run_end[hb];
-        const int z0 = max(bz - reach[2], 0), z1 = min(bz + reach[2], mm[2] - 1);
-        const int y0 = max(by - reach[1], 0), y1 = min(by + reach[1], mm[1] - 1);
-        const int x0 = max(bx - reach[0], 0), x1 = min(bx + reach[0], mm[0] - 1);
-        for (int az = z0; az <= z1; az++) {
-          const int gz = max(abs(az - bz) - 1, 0);
-          for (int ay = y0; ay <= y1; ay++) {
-            const int gy = max(abs(ay - by) - 1, 0);
-            for (int ax = x0; ax <= x1; ax++) {
-              const int gx = max(abs(ax - bx) - 1, 0);
-              const double g2 = (double)(gx * gx) * V.s2[0] + (double)(gy * gy) * V.s2[1] +
-                                (double)(gz * gz) * V.s2[2];
-              if (g2 > R.prune2) continue;
-              const uint32_t ia = amap[(az * mm[1] + ay) * mm[0] + ax];
-              if (ia == 0xFFFF) continue;
-              if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
-              const uint32_t ha = ahead[ia];
-              if (agr[ia] == bgr[ib] || runs_link(pos4, ha, run_end[ha], hb, eb, R, tests, rechecks))
-                lunion(lp, ia, na + ib);
-            }
-          }
+      // each neighbour run against the home runs: every pair, or the home
+      // sub-cells within reach through the map when that window is smaller
+      const int wAB = min(2 * reach[0] + 1, mm[0]) * min(2 * reach[1] + 1, mm[1]) *
+                      min(2 * reach[2] + 1, mm[2]);
+      if (na <= wAB) {
+        const uint32_t npair = (uint32_t)na * (uint32_t)nb;
+        for (uint32_t p = threadIdx.x; p < npair; p += blockDim.x) {
+          const uint32_t ib = p / (uint32_t)na, ia = p - ib * (uint32_t)na;
+          const uint32_t ra = arel[ia], rb = brel[ib];
+          if (!gap_ok(rel_x(rb) - rel_x(ra), rel_y(rb) - rel_y(ra), rel_z(rb) - rel_z(ra)))
+            continue;
+          if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
+          const uint32_t ha = ahead[ia], hb = bhead[ib];
+          if (agr[ia] == bgr[ib] ||
+              runs_link(pos4, ha, run_end[ha], hb, run_end[hb], R, tests, rechecks))
+            lunion(lp, ia, na + ib);
+        }
+      } else {
+        for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
+          const uint32_t rb = brel[ib];
+          const int bx = rel_x(rb), by = rel_y(rb), bz = rel_z(rb);
+          const uint32_t hb = bhead[ib], eb = run_end[hb];
+          const int z0 = max(bz - reach[2], 0), z1 = min(bz + reach[2], mm[2] - 1);
+          const int y0 = max(by - reach[1], 0), y1 = min(by + reach[1], mm[1] - 1);
+          const int x0 = max(bx - reach[0], 0), x1 = min(bx + reach[0], mm[0] - 1);
+          for (int az = z0; az <= z1; az++)
+            for (int ay = y0; ay <= y1; ay++)
+              for (int ax = x0; ax <= x1; ax++) {
+                if (!gap_ok(ax - bx, ay - by, az - bz)) continue;
+                const uint32_t ia = amap[(az * mm[1] + ay) * mm[0] + ax];
+                if (ia == 0xFFFF) continue;
+                if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
+                const uint32_t ha = ahead[ia];
+                if (agr[ia] == bgr[ib] ||
+                    runs_link(pos4, ha, run_end[ha], hb, eb, R, tests, rechecks))
+                  lunion(lp, ia, na + ib);
+              }
         }
       }
       __syncthreads();
@@ -1340,7 +1331,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   const uint32_t flags = opts ? opts->flags : 0u;
   double band = (opts && opts->band > 0) ? opts->band : 1e-5;
   if (!(band >= 1e-6 && band <= 0.1)) return FOF_E_ARG;
-  const uint32_t T = std::min(kMaxT, (opts && opts->dense_threshold) ? opts->dense_threshold : 12u);
+  const uint32_t T = std::min(kMaxT, (opts && opts->dense_threshold) ? opts->dense_threshold : kDefaultT);
   const bool out_dev = (flags & FOF_OPT_OUT_DEVICE) != 0;
   const bool want_ring = next_out && !(flags & FOF_OPT_NO_RING);
   const bool want_halos = !(flags & FOF_OPT_NO_HALOS);
@@ -1753,6 +1744,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     stats->nhalos = H;
     stats->nhalos_owned = hc[3];
     stats->sparse = sparse ? 1u : 0u;
+    stats->dense_threshold = T;
     if (timing) {
       float ms;
       for (int k = 0; k < FOF_NPHASE - 1; k++) {

@@ -107,7 +107,7 @@ typedef struct fof_opts_v1 {
   double band;              /* relative FP32 band around l^2 that is re-decided
                                in FP64; 0 selects 1e-5; must lie in [1e-6,0.1] */
   uint32_t dense_threshold; /* cells with at least this many particles are
-                               linked at sub-cell level; 0 = default (12),
+                               linked at sub-cell level; 0 = default (96),
                                values above 512 act as 512                   */
   uint32_t reserved;
 } fof_opts_v1;
@@ -156,7 +156,7 @@ typedef struct fof_stats_v1 {
   uint64_t nhalos;        /* groups passing the size cut                     */
   uint64_t nhalos_owned;  /* of those, owned (Checkhalo)                     */
   uint32_t sparse;        /* 1 if occupied-cell addressing was used          */
-  uint32_t reserved;
+  uint32_t dense_threshold; /* the dense-cell threshold in effect            */
   double phase_ms[FOF_NPHASE];
   double link_kernel_ms;  /* spans 5 + 6: the pair-test kernels              */
 } fof_stats_v1;

@@ -265,7 +265,7 @@ def test_variable_linking_length(ctx, seed):
     dom = L.domain([-R] * 3, [R] * 3, [-R] * 3, [R] * 3, [lc] * 3, 0.0, 5, vl)
     ovl = O.varlink([0.0, 0.0, 0.0], r2max, tab)
     od = odomain(dom)
-    lab, nxt, halos, st = gpu_link(ctx, P, dom)
+    lab, nxt, halos, st = gpu_link(ctx, P, dom, T=12)  # small clumps: low threshold
     olab, _ = O.link_uf(P["Pos"], od, ovl)
     assert np.array_equal(lab, olab)
     assert np.array_equal(O.link_brute(P["Pos"][:6000], od, ovl),
@@ -309,7 +309,7 @@ def test_sparse_variable_linking_length(ctx):
     tab = (0.2 * lc * (0.8 + (r / R) ** 0.8 * 0.45)) ** 2
     vl = L.VarLinkTable([0.0, 0.0, 0.0], r2max, tab)
     dom = L.domain([-R] * 3, [R] * 3, [-R] * 3, [R] * 3, [lc] * 3, 0.0, 5, vl)
-    lab, _, _, st = gpu_link(ctx, P, dom, flags=L.OPT_SPARSE)
+    lab, _, _, st = gpu_link(ctx, P, dom, flags=L.OPT_SPARSE, T=12)
     olab, _ = O.link_uf(P["Pos"], odomain(dom), O.varlink([0.0, 0.0, 0.0], r2max, tab))
     assert np.array_equal(lab, olab)
     assert st.ndense > 0

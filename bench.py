@@ -125,7 +125,20 @@ def candidate_pairs(recs_u8, vols, T):
     return dp, sp, nd
 
 
-def roofline_link(stats, L, recs_u8, vols):
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE and WRITE_SIZE passes, scaled by a
+    4 GiB copy), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_hbm.txt")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
+def roofline_link(stats, L, recs_u8, vols, workload=None):
     """Roofline of the dominant link kernel of a timed call: algorithmic FP32
     flops (8 per candidate pair of the reference sweep) over the kernel's
     device time from the library's HIP events on its stream."""
@@ -136,6 +149,7 @@ def roofline_link(stats, L, recs_u8, vols):
     else:
         kern, ms, pairs = "k_link_sparse", ph["link_sparse"], sp
     tf = pairs * 8 / (ms * 1e-3) / 1e12
+    traffic, src = profiled_traffic(kern, workload) if workload else (None, None)
     return {
         "kernel": kern,
         "bound": "mfma",
@@ -144,7 +158,8 @@ def roofline_link(stats, L, recs_u8, vols):
         "peak": 157.3,
         "unit": "TFLOP/s",
         "frac": tf / 157.3,
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": src,
         "kernel_ms": ms,
         "algorithmic_pairs": pairs,
         "flop_per_pair": 8,
@@ -322,7 +337,8 @@ def main():
     same_groups = bool(fp1.shape == fpB.shape and torch.equal(fp1, fpB))
     log(f"owned halos plain {fp1.shape[0]} batched {fpB.shape[0]} identical membership: {same_groups}")
 
-    roof = roofline_link(stB, L, DB, VB)
+    roof = roofline_link(stB, L, DB, VB,
+                         "box_1024" if args.workload == "box" and args.pside == 1024 and args.g == 4 else None)
     log("roofline:", roof)
 
     cpu = None

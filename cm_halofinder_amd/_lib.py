@@ -105,9 +105,10 @@ HALO_DTYPE = np.dtype(
 )
 assert HALO_DTYPE.itemsize == C.sizeof(Halo) == 64
 
-NPHASE = 16
+NPHASE = 17
 PHASES = ("keys", "scan", "scatter", "uf_init", "subsort", "link_sparse", "link_dense",
-          "flatten", "label", "compact", "ring_sort", "ring", "halo_list", "halo_sums", "output", "total")
+          "flatten", "label", "compact", "ring_sort", "ring", "halo_list", "halo_sums", "output",
+          "link_dense_wave", "total")
 
 
 class Stats(C.Structure):

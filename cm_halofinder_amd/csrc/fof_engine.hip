@@ -292,6 +292,10 @@ __device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t t
   }
 }
 
+// Dense cells with at least this many particles are linked by a 256-thread
+// workgroup each, smaller ones by a single wave each.
+constexpr uint32_t kBigCell = 256;
+
 // ---------------------------------------------------------------------------
 // 1: keys + histogram
 // ---------------------------------------------------------------------------
@@ -330,12 +334,15 @@ __global__ __launch_bounds__(256) void k_keys(const uint8_t *__restrict__ base, 
 __global__ __launch_bounds__(256) void k_dense_list(uint32_t C, const uint32_t *__restrict__ cnt,
                                                     const VolDev *__restrict__ vols, int nvol,
                                                     uint32_t T, uint32_t *__restrict__ dense,
-                                                    uint32_t *__restrict__ ndense) {
+                                                    uint32_t *__restrict__ ndense,
+                                                    uint32_t *__restrict__ dbig,
+                                                    uint32_t *__restrict__ nbig) {
   const int lane = threadIdx.x & 63;
   for (uint32_t c0 = (blockIdx.x * blockDim.x) & ~63u; c0 < C; c0 += gridDim.x * blockDim.x) {
     const uint32_t c = c0 + lane + (threadIdx.x & ~63u);
     bool is = false;
-    if (c < C && cnt[c] >= T) {
+    const uint32_t n = (c < C) ? cnt[c] : 0u;
+    if (n >= T) {
       // volume of cell c (cells are laid out volume after volume)
       int lo = 0, hi = nvol - 1;
       while (lo < hi) {
@@ -345,12 +352,21 @@ __global__ __launch_bounds__(256) void k_dense_list(uint32_t C, const uint32_t *
       }
       is = vols[lo].m[0] > 0;
     }
-    const unsigned long long b = __ballot(is);
+    // two lists: cells below kBigCell particles (one wave each) and the rest
+    const bool big = is && n >= kBigCell;
+    const bool sml = is && !big;
+    const unsigned long long b = __ballot(sml), bb = __ballot(big);
     if (b) {
       uint32_t base = 0;
       if (lane == 0) base = atomicAdd(ndense, (uint32_t)__popcll(b));
       base = __shfl(base, 0, 64);
-      if (is) dense[base + __popcll(b & ((1ull << lane) - 1))] = c;
+      if (sml) dense[base + __popcll(b & ((1ull << lane) - 1))] = c;
+    }
+    if (bb) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(nbig, (uint32_t)__popcll(bb));
+      base = __shfl(base, 0, 64);
+      if (big) dbig[base + __popcll(bb & ((1ull << lane) - 1))] = c;
     }
   }
 }
@@ -446,7 +462,7 @@ __global__ __launch_bounds__(256) void k_cells_finish(
     unsigned long long *__restrict__ hkey, uint32_t *__restrict__ hval,
     uint32_t *__restrict__ cell_of, const VolDev *__restrict__ vols,
     const uint32_t *__restrict__ vbeg, int nvol, uint32_t T, uint32_t *__restrict__ dense,
-    uint32_t *__restrict__ ndense) {
+    uint32_t *__restrict__ ndense, uint32_t *__restrict__ dbig, uint32_t *__restrict__ nbig) {
   const uint32_t U = *dU;
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < U;
        c += gridDim.x * blockDim.x) {
@@ -463,7 +479,10 @@ __global__ __launch_bounds__(256) void k_cells_finish(
       i = (i + 1) & G.hmask;
     }
     const VolDev &V = vols[vol_of(vbeg, nvol, s)];
-    if (V.m[0] > 0 && e - s >= T) dense[atomicAdd(ndense, 1u)] = c;
+    if (V.m[0] > 0 && e - s >= T) {
+      if (e - s >= kBigCell) dbig[atomicAdd(nbig, 1u)] = c;
+      else dense[atomicAdd(ndense, 1u)] = c;
+    }
   }
 }
 
@@ -707,10 +726,9 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
 // 6: dense cells -- one workgroup per cell, run pairs with early exit.
 // ---------------------------------------------------------------------------
 // Run-table capacities.  The home cell has one run per occupied sub-cell,
-// at most 12^3; neighbour runs (dense neighbours) or particles (sparse
-// neighbours) are taken kMaxRunsB at a time.
+// at most 12^3 (and at most one per particle); neighbour runs (dense
+// neighbours) or particles (sparse neighbours) are taken a chunk at a time.
 constexpr int kMaxRuns = 1728;
-constexpr int kMaxRunsB = 512;  // neighbour items per chunk
 constexpr uint32_t kMaxT = 512;
 // Default dense threshold: the sweep over clumpy 1024^3 boxes (b = 0.2,
 // about one particle per cell) is flat within 5% from 64 to 256.
@@ -777,7 +795,36 @@ __device__ __forceinline__ bool runs_link(const float4 *__restrict__ pos4, uint3
   return false;
 }
 
-__global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__ dense,
+// Run pairs with more than kBigPair particle pairs are queued in LDS and
+// tested by a whole wave: 64 particle pairs per step, ballot exit.
+constexpr uint32_t kBigPair = 24;
+
+__device__ __forceinline__ bool runs_link_wave(const float4 *__restrict__ pos4, uint32_t ha,
+                                               uint32_t sa, uint32_t hb, uint32_t sb,
+                                               const Rule &R, uint32_t &tests,
+                                               uint32_t &rechecks) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t total = sa * sb;
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t idx = base + lane;
+    bool l = false;
+    if (idx < total) {
+      const uint32_t i = idx / sb, j = idx - i * sb;
+      l = pair_linked(pos4[ha + i], pos4[hb + j], R, rechecks);
+      tests++;
+    }
+    if (__ballot(l)) return true;
+  }
+  return false;
+}
+
+// Home-cell sub-cell coordinates (0..11) packed 4 bits per axis.
+__device__ __forceinline__ uint16_t pack_home(int x, int y, int z) {
+  return (uint16_t)(x | (y << 4) | (z << 8));
+}
+
+template <int NT, int MAXRUNS, int MAXRUNSB, int BIGQ>
+__global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ dense,
                                                     const uint32_t *__restrict__ ndense,
                                                     const float4 *__restrict__ pos4,
                                                     const Grid G,
@@ -792,19 +839,22 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
                                                     int nvol,
                                                     unsigned long long *__restrict__ ctr,
                                                     uint32_t *__restrict__ overflow) {
-  __shared__ uint32_t ahead[kMaxRuns], arel[kMaxRuns], agr[kMaxRuns];
-  __shared__ uint32_t bhead[kMaxRunsB], brel[kMaxRunsB], bgr[kMaxRunsB];
-  __shared__ uint32_t lp[kMaxRuns + kMaxRunsB];
-  __shared__ uint16_t amap[kMaxRuns];  // sub-cell of A -> run slot
+  __shared__ uint32_t ahead[MAXRUNS], agr[MAXRUNS];
+  __shared__ uint16_t arel[MAXRUNS];
+  __shared__ uint32_t bhead[MAXRUNSB], brel[MAXRUNSB], bgr[MAXRUNSB];
+  __shared__ uint32_t lp[MAXRUNS + MAXRUNSB];
+  __shared__ uint16_t amap[kMaxRuns];  // sub-cell of A -> run slot (<= 12^3 sub-cells)
+  __shared__ uint32_t bigq[BIGQ];
   __shared__ uint32_t nbS[13], nbOff[14], nbTab[13];
   __shared__ int nbLo[13][3], nbHi[13][3];
   __shared__ uint8_t nbDense[13];
-  __shared__ uint32_t cnt;
+  __shared__ uint32_t cnt, qn;
   uint32_t tests = 0, rechecks = 0;
+  const uint32_t lane = threadIdx.x & 63;
   const uint32_t nd = *ndense;
   for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
     const uint32_t c = dense[d];
-    const uint32_t sA = G.start[c], eA = G.start[c + 1];
+    const uint32_t sA = G.start[c];
     const VolDev &V = vols[vol_of(vbeg, nvol, sA)];
     const unsigned long long f0 = fine[sA];
     const int mm[3] = {V.m[0], V.m[1], V.m[2]};
@@ -817,7 +867,8 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
       reach[a] = min((int)floor(sqrt(R.prune2 / V.s2[a])) + 1, mm[a]);
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = 0xFFFF;
     // neighbour table: the 13 forward cells of src_v2/FOF.c:122-153, their
-    // particle ranges and the window of their runs that can reach A
+    // items (runs of dense neighbours, particles of sparse ones) and the
+    // window of sub-cells that can reach A
     if (threadIdx.x < 13) {
       const int k = threadIdx.x;
       int dd[3] = {0, 0, 0};
@@ -830,7 +881,6 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
         const uint32_t cb = cell_index(G, V, bx, by, bz);
         if (cb != kNone) sB = G.start[cb], eB = G.start[cb + 1];
       }
-      // items of a neighbour: its runs (dense, from the run table) or particles
       const bool dn = (eB - sB >= T);
       nbS[k] = sB;
       nbDense[k] = dn ? 1 : 0;
@@ -843,22 +893,22 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
         nbLo[k][a] = lo, nbHi[k][a] = hi;
       }
     }
-    // the home cell's runs (one per occupied sub-cell)
-    if (threadIdx.x == 0) cnt = 0;
+    if (threadIdx.x == 0) cnt = 0, qn = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
       nbOff[0] = 0;
       for (int k = 0; k < 13; k++) nbOff[k + 1] += nbOff[k];
     }
+    // the home cell's runs (one per occupied sub-cell) from its run table
     const uint32_t atab = rtoff[sA];
     const uint32_t nruns = rtab[atab];
     for (uint32_t t = threadIdx.x; t < nruns; t += blockDim.x) {
       const uint32_t j = rtab[atab + 1 + t];
       const unsigned long long f = fine[j];
       const int rx = fine_x(f) - ox, ry = fine_y(f) - oy, rz = fine_z(f) - oz;
-      const uint32_t k = atomicAdd(&cnt, 1u);  // <= nbins <= kMaxRuns
+      const uint32_t k = atomicAdd(&cnt, 1u);  // runs <= min(particles, sub-cells) <= MAXRUNS
       ahead[k] = j;
-      arel[k] = pack_rel(rx, ry, rz);
+      arel[k] = pack_home(rx, ry, rz);
       agr[k] = uf_find(par, j);  // snapshot of the global root
       lp[k] = k;
       amap[(rz * mm[1] + ry) * mm[0] + rx] = (uint16_t)k;
@@ -873,6 +923,41 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
       const int gx = max(abs(dx) - 1, 0), gy = max(abs(dy) - 1, 0), gz = max(abs(dz) - 1, 0);
       return (float)(gx * gx) * gs0 + (float)(gy * gy) * gs1 + (float)(gz * gz) * gs2 <= gp2;
     };
+    // a run pair (LDS slots sa < sb, heads ha, hb) not yet joined: small
+    // pairs are tested here, big ones go to the wave queue
+    auto test_pair = [&](uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb) {
+      const uint32_t ea = run_end[ha], eb = run_end[hb];
+      if ((ea - ha) * (eb - hb) > kBigPair) {
+        const uint32_t q = atomicAdd(&qn, 1u);
+        if (q < (uint32_t)BIGQ) {
+          bigq[q] = (sa << 16) | sb;
+          return;
+        }
+      }
+      if (runs_link(pos4, ha, ea, hb, eb, R, tests, rechecks)) lunion(lp, sa, sb);
+    };
+    // the queued pairs, one wave each: skip when joined in LDS or under the
+    // same global root now, else a wave-wide test
+    auto drain = [&]() {
+      __syncthreads();
+      const uint32_t nq = min(qn, (uint32_t)BIGQ);
+      for (uint32_t k = threadIdx.x >> 6; k < nq; k += blockDim.x >> 6) {
+        const uint32_t e = bigq[k];
+        const uint32_t sa = e >> 16, sb = e & 0xFFFFu;
+        const uint32_t ha = ahead[sa], hb = sb < (uint32_t)na ? ahead[sb] : bhead[sb - na];
+        uint32_t v = 0;
+        if (lane == 0) {
+          v = (lfind(lp, sa) == lfind(lp, sb)) ? 2u : 0u;
+          if (!v && uf_find(par, ha) == uf_find(par, hb)) v = 1u;
+        }
+        v = __shfl(v, 0, 64);
+        if (v == 2u) continue;
+        const bool link = (v == 1u) || runs_link_wave(pos4, ha, run_end[ha] - ha, hb,
+                                                      run_end[hb] - hb, R, tests, rechecks);
+        if (link && lane == 0) lunion(lp, sa, sb);
+      }
+      __syncthreads();
+    };
     // (A, A): every run pair, or forward sub-cell offsets through the map
     // when the cell has more runs than the offset window
     const int wAA = (reach[2] + 1) * (2 * reach[1] + 1) * (2 * reach[0] + 1);
@@ -882,18 +967,18 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
         const uint32_t ia = p / (uint32_t)na, ib = p - ia * (uint32_t)na;
         if (ib <= ia) continue;
         const uint32_t ra = arel[ia], rb = arel[ib];
-        if (!gap_ok(rel_x(rb) - rel_x(ra), rel_y(rb) - rel_y(ra), rel_z(rb) - rel_z(ra))) continue;
+        if (!gap_ok((int)(rb & 15) - (int)(ra & 15), (int)((rb >> 4) & 15) - (int)((ra >> 4) & 15),
+                    (int)(rb >> 8) - (int)(ra >> 8)))
+          continue;
         if (lfind(lp, ia) == lfind(lp, ib)) continue;
-        const uint32_t ha = ahead[ia], hb = ahead[ib];
-        // joined elsewhere (snapshot of the global roots), or linked here
-        if (agr[ia] == agr[ib] || runs_link(pos4, ha, run_end[ha], hb, run_end[hb], R, tests, rechecks))
-          lunion(lp, ia, ib);
+        // joined elsewhere (snapshot of the global roots), or test
+        if (agr[ia] == agr[ib]) lunion(lp, ia, ib);
+        else test_pair(ia, ib, ahead[ia], ahead[ib]);
       }
     } else {
       for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
         const uint32_t ra = arel[ia];
-        const int ax = rel_x(ra), ay = rel_y(ra), az = rel_z(ra);
-        const uint32_t ha = ahead[ia], ea = run_end[ha];
+        const int ax = ra & 15, ay = (ra >> 4) & 15, az = ra >> 8;
         for (int dz = 0; dz <= reach[2]; dz++) {
           const int bz = az + dz;
           if (bz >= mm[2]) break;
@@ -909,25 +994,27 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
               const uint32_t ib = amap[(bz * mm[1] + by) * mm[0] + bxx];
               if (ib == 0xFFFF) continue;
               if (lfind(lp, ia) == lfind(lp, ib)) continue;
-              const uint32_t hb = ahead[ib];
-              if (agr[ia] == agr[ib] || runs_link(pos4, ha, ea, hb, run_end[hb], R, tests, rechecks))
-                lunion(lp, ia, ib);
+              if (agr[ia] == agr[ib]) lunion(lp, ia, ib);
+              else if (ia < (int)ib) test_pair(ia, ib, ahead[ia], ahead[ib]);
+              else test_pair(ib, ia, ahead[ib], ahead[ia]);
             }
           }
         }
       }
     }
-    __syncthreads();
+    drain();
     // publish the home cell's merges before the neighbour pass
     for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
       const uint32_t r = lfind(lp, ia);
       if (r != (uint32_t)ia) uf_union(par, ahead[ia], ahead[r]);
     }
-    // (A, B): the neighbours' particles in chunks of kMaxRunsB
+    // (A, B): the neighbours' items in chunks of MAXRUNSB
     const uint32_t ntot = nbOff[13];
-    for (uint32_t c0 = 0; c0 < ntot; c0 += kMaxRunsB) {
-      const uint32_t c1 = min(ntot, c0 + (uint32_t)kMaxRunsB);
-      if (threadIdx.x == 0) cnt = 0;
+    const int wAB = min(2 * reach[0] + 1, mm[0]) * min(2 * reach[1] + 1, mm[1]) *
+                    min(2 * reach[2] + 1, mm[2]);
+    for (uint32_t c0 = 0; c0 < ntot; c0 += MAXRUNSB) {
+      const uint32_t c1 = min(ntot, c0 + (uint32_t)MAXRUNSB);
+      if (threadIdx.x == 0) cnt = 0, qn = 0;
       __syncthreads();
       for (uint32_t g = c0 + threadIdx.x; g < c1; g += blockDim.x) {
         int q = 0;
@@ -947,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
         if (rx < nbLo[q][0] || rx > nbHi[q][0] || ry < nbLo[q][1] || ry > nbHi[q][1] ||
             rz < nbLo[q][2] || rz > nbHi[q][2])
           continue;
-        const uint32_t k = atomicAdd(&cnt, 1u);  // < kMaxRunsB: one run per particle at most
+        const uint32_t k = atomicAdd(&cnt, 1u);  // < MAXRUNSB: one per item at most
         bhead[k] = j;
         brel[k] = pack_rel(rx, ry, rz);
         bgr[k] = uf_find(par, j);
@@ -957,26 +1044,23 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
       const int nb = (int)cnt;
       // each neighbour run against the home runs: every pair, or the home
       // sub-cells within reach through the map when that window is smaller
-      const int wAB = min(2 * reach[0] + 1, mm[0]) * min(2 * reach[1] + 1, mm[1]) *
-                      min(2 * reach[2] + 1, mm[2]);
       if (na <= wAB) {
         const uint32_t npair = (uint32_t)na * (uint32_t)nb;
         for (uint32_t p = threadIdx.x; p < npair; p += blockDim.x) {
           const uint32_t ib = p / (uint32_t)na, ia = p - ib * (uint32_t)na;
           const uint32_t ra = arel[ia], rb = brel[ib];
-          if (!gap_ok(rel_x(rb) - rel_x(ra), rel_y(rb) - rel_y(ra), rel_z(rb) - rel_z(ra)))
+          if (!gap_ok(rel_x(rb) - (int)(ra & 15), rel_y(rb) - (int)((ra >> 4) & 15),
+                      rel_z(rb) - (int)(ra >> 8)))
             continue;
           if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
-          const uint32_t ha = ahead[ia], hb = bhead[ib];
-          if (agr[ia] == bgr[ib] ||
-              runs_link(pos4, ha, run_end[ha], hb, run_end[hb], R, tests, rechecks))
-            lunion(lp, ia, na + ib);
+          if (agr[ia] == bgr[ib]) lunion(lp, ia, na + ib);
+          else test_pair(ia, na + ib, ahead[ia], bhead[ib]);
         }
       } else {
         for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
           const uint32_t rb = brel[ib];
           const int bx = rel_x(rb), by = rel_y(rb), bz = rel_z(rb);
-          const uint32_t hb = bhead[ib], eb = run_end[hb];
+          const uint32_t hb = bhead[ib];
           const int z0 = max(bz - reach[2], 0), z1 = min(bz + reach[2], mm[2] - 1);
           const int y0 = max(by - reach[1], 0), y1 = min(by + reach[1], mm[1] - 1);
           const int x0 = max(bx - reach[0], 0), x1 = min(bx + reach[0], mm[0] - 1);
@@ -987,14 +1071,12 @@ __global__ __launch_bounds__(256) void k_link_dense(const uint32_t *__restrict__
                 const uint32_t ia = amap[(az * mm[1] + ay) * mm[0] + ax];
                 if (ia == 0xFFFF) continue;
                 if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
-                const uint32_t ha = ahead[ia];
-                if (agr[ia] == bgr[ib] ||
-                    runs_link(pos4, ha, run_end[ha], hb, eb, R, tests, rechecks))
-                  lunion(lp, ia, na + ib);
+                if (agr[ia] == bgr[ib]) lunion(lp, ia, na + ib);
+                else test_pair(ia, na + ib, ahead[ia], hb);
               }
         }
       }
-      __syncthreads();
+      drain();
       // commit: a neighbour run joined in LDS joins its (home-cell) root's run
       for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
         const uint32_t r = lfind(lp, na + ib);
@@ -1457,7 +1539,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if ((rc = ensure(W.label, N * 4 + 64))) return rc;
   const uint64_t q4 = (N + 63) / 64 * 64;  // u32 entries per sort array
   if ((rc = ensure(W.sortbuf, 16 * q4 + 256))) return rc;
-  if ((rc = ensure(W.dense, ndense_cap * 4 + 64))) return rc;
+  if ((rc = ensure(W.dense, 2 * ndense_cap * 4 + 64))) return rc;  // two lists
   size_t tmpb = std::max(std::max(scan_tmp_bytes(Csz + 1), sort_tmp_bytes(N)), compact_tmp_bytes(N));
   if ((rc = ensure(W.tmp, tmpb))) return rc;
   if ((rc = ensure(W.vols, nvol * sizeof(VolDev) + vt_entries * sizeof(double) + 256))) return rc;
@@ -1522,7 +1604,10 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   uint32_t *sk0 = static_cast<uint32_t *>(W.sortbuf.ptr);
   uint32_t *sv0 = sk0 + q4, *sk1 = sk0 + 2 * q4, *sv1 = sk0 + 3 * q4;
   uint32_t *dense = static_cast<uint32_t *>(W.dense.ptr);
-  uint32_t *small = static_cast<uint32_t *>(W.small.ptr);  // [0] err [1] ndense [2] M [3] H [4] overflow
+  uint32_t *dbig = dense + ndense_cap;  // cells of kBigCell particles or more
+  // small: [0] error [1] small dense cells [2] M [3] H [4] overflow [5] U
+  //        [6] run-table fill [7] big dense cells
+  uint32_t *small = static_cast<uint32_t *>(W.small.ptr);
   unsigned long long *ctr = reinterpret_cast<unsigned long long *>(small + 16);  // 8 counters
   const uint32_t NN = (uint32_t)N;
   const uint32_t stride = P->stride, offpos = P->off_pos, offvel = P->off_vel;
@@ -1561,7 +1646,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   G.sparse = sparse ? 1 : 0;
   G.volshift = volshift;
   uint32_t hsmall[8];
-  uint32_t ndense = 0;
+  uint32_t ndense = 0, nbig = 0;  // dense cells below / from kBigCell particles
   if (!sparse) {
     // 1-3: keys + histogram, scan, counting-sort scatter
     FOF_HIP(hipMemsetAsync(cnt, 0, C * 4, st));
@@ -1569,13 +1654,15 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     k_keys<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, key, cnt, T,
                                dense, small + 1, small + 0);
     FOF_HIP(hipGetLastError());
-    k_dense_list<<<grid_for(C), 256, 0, st>>>((uint32_t)C, cnt, dV, (int)nvol, T, dense, small + 1);
+    k_dense_list<<<grid_for(C), 256, 0, st>>>((uint32_t)C, cnt, dV, (int)nvol, T, dense, small + 1,
+                                              dbig, small + 7);
     FOF_HIP(hipGetLastError());
     if ((rc = te(0))) return rc;
     FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
     FOF_HIP(hipStreamSynchronize(st));
     if (hsmall[0]) return FOF_E_DOMAIN;
     ndense = hsmall[1];
+    nbig = hsmall[7];
     if ((rc = tb(1))) return rc;
     if ((rc = scan_u32(cnt, start, C, W.tmp.ptr, st))) return rc;
     if ((rc = te(1))) return rc;
@@ -1622,11 +1709,13 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     G.hval = hvv;
     G.hmask = hsz - 1;
     k_cells_finish<<<grid_for(U), 256, 0, st>>>(small + 5, NN, start, par, runend, G, hk, hvv, label,
-                                                dV, dVB, (int)nvol, T, dense, small + 1);
+                                                dV, dVB, (int)nvol, T, dense, small + 1, dbig,
+                                                small + 7);
     FOF_HIP(hipGetLastError());
     FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
     FOF_HIP(hipStreamSynchronize(st));
     ndense = hsmall[1];
+    nbig = hsmall[7];
     if (stats) stats->ncells = U;
   }
   if ((rc = tb(3))) return rc;
@@ -1634,13 +1723,17 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   FOF_HIP(hipGetLastError());
   if ((rc = te(3))) return rc;
   // 4: sub-cell sort of dense cells (tmp4 aliases the sort buffers s0..s3)
-  const uint32_t gD = std::max(1u, std::min(ndense, 8192u));
-  if (ndense) {
+  if (ndense + nbig) {
     if ((rc = tb(4))) return rc;
-    k_subsort<<<gD, 256, 0, st>>>(dense, small + 1, start, dV, dVB, (int)nvol, pos4,
-                                  reinterpret_cast<float4 *>(sk0), par, runend, fine, rtoff, rtab,
-                                  small + 6);
-    FOF_HIP(hipGetLastError());
+    const uint32_t lists[2] = {ndense, nbig};
+    uint32_t *lptr[2] = {dense, dbig};
+    for (int l = 0; l < 2; l++) {
+      if (!lists[l]) continue;
+      k_subsort<<<std::min(lists[l], 8192u), 256, 0, st>>>(
+          lptr[l], small + (l ? 7 : 1), start, dV, dVB, (int)nvol, pos4,
+          reinterpret_cast<float4 *>(sk0), par, runend, fine, rtoff, rtab, small + 6);
+      FOF_HIP(hipGetLastError());
+    }
     if ((rc = te(4))) return rc;
   }
   // 5-6: linking
@@ -1648,13 +1741,21 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   k_link_sparse<<<gN, 256, 0, st>>>(NN, pos4, G, runend, fine, par, dV, dVB, (int)nvol, T, ctr);
   FOF_HIP(hipGetLastError());
   if ((rc = te(5))) return rc;
-  if (ndense) {
+  if (nbig) {
     if ((rc = tb(6))) return rc;
-    k_link_dense<<<gD, 256, 0, st>>>(dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par,
-                                     dV, dVB,
-                                     (int)nvol, ctr, small + 4);
+    k_link_dense<256, kMaxRuns, 512, 768><<<std::min(nbig, 8192u), 256, 0, st>>>(
+        dbig, small + 7, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
+        small + 4);
     FOF_HIP(hipGetLastError());
     if ((rc = te(6))) return rc;
+  }
+  if (ndense) {
+    if ((rc = tb(15))) return rc;
+    k_link_dense<64, (int)kBigCell, 128, 152><<<std::min(ndense, 65536u), 64, 0, st>>>(
+        dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
+        small + 4);
+    FOF_HIP(hipGetLastError());
+    if ((rc = te(15))) return rc;
   }
   // 7: union-find finish (minorig reuses runend, hsize reuses key)
   uint32_t *minorig = runend, *hsize = key;
@@ -1736,7 +1837,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   FOF_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
   FOF_HIP(hipStreamSynchronize(st));
   if (stats) {
-    stats->ndense = ndense;
+    stats->ndense = ndense + nbig;
     stats->pair_tests = hc[0];
     stats->rechecks = hc[1];
     stats->ngroups = hc[2];
@@ -1756,7 +1857,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
         FOF_HIP(hipEventElapsedTime(&ms, S.ev[2 * first_span], S.ev[2 * last_span + 1]));
         stats->phase_ms[FOF_NPHASE - 1] = ms;
       }
-      stats->link_kernel_ms = stats->phase_ms[5] + stats->phase_ms[6];
+      stats->link_kernel_ms = stats->phase_ms[5] + stats->phase_ms[6] + stats->phase_ms[15];
     }
   }
   return 0;

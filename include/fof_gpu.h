@@ -136,12 +136,14 @@ typedef struct fof_halo_v1 {
  *   0 cell keys (k_keys)            1 cell starts (scan) or key sort (sparse)
  *   2 counting-sort scatter         3 union-find init
  *   4 sub-cell sort of dense cells  5 pair tests, sparse cells
- *   6 pair tests, dense cells       7 union-find flatten + group minima
+ *   6 pair tests, dense cells of    7 union-find flatten + group minima
+ *     256 particles or more
  *   8 labels                        9 compaction of kept groups
  *  10 radix sort by label          11 ring
  *  12 halo list                    13 halo sums
- *  14 output copies                15 total, first span to last            */
-#define FOF_NPHASE 16
+ *  14 output copies                15 pair tests, dense cells below 256
+ *  16 total, first span to last                                             */
+#define FOF_NPHASE 17
 typedef struct fof_stats_v1 {
   uint32_t version;
   uint32_t nvol;
@@ -158,7 +160,7 @@ typedef struct fof_stats_v1 {
   uint32_t sparse;        /* 1 if occupied-cell addressing was used          */
   uint32_t dense_threshold; /* the dense-cell threshold in effect            */
   double phase_ms[FOF_NPHASE];
-  double link_kernel_ms;  /* spans 5 + 6: the pair-test kernels              */
+  double link_kernel_ms;  /* spans 5 + 6 + 15: the pair-test kernels         */
 } fof_stats_v1;
 
 typedef struct fof_ctx fof_ctx;

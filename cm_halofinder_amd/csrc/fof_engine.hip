@@ -6,16 +6,19 @@
 // :378-430, :797-847) derive from the ring.
 //
 // Pipeline (one launch per phase for all volumes of a batch):
-//   1 k_keys       reference cell of every particle (src_v2/FOF.c:79-83),
-//                  histogram, dense-cell list
-//   2 scan         cell start offsets
-//   3 k_scatter    counting sort -> float4 {x,y,z,index} in cell order
-//   4 k_subsort    cells with >= T particles: counting sort by sub-cell;
-//                  sub-cells have side <= l/sqrt(3), so each is a clique and
-//                  its particles start in one union-find set ("runs")
+//   1 k_part_count   reference cell of every particle (src_v2/FOF.c:79-83),
+//                    per-tile histograms of cell buckets
+//   2 scan           (bucket, tile) offsets
+//   3 k_part_scatter, k_bucket_sort   partition by bucket, then a counting
+//                    sort per bucket -> cell starts, dense-cell lists and
+//                    float4 {x,y,z,index} in cell order
+//   4 k_subsort      cells with >= T particles: counting sort by sub-cell;
+//                    sub-cells have side <= l/sqrt(3), so each is a clique and
+//                    its particles start in one union-find set ("runs")
 //   5 k_link_sparse  one thread per particle of a sparse cell: own cell and
-//                  the 13 forward cells (src_v2/FOF.c:112-153)
-//   6 k_link_dense   one workgroup per dense cell: run pairs with early exit
+//                    the 13 forward cells (src_v2/FOF.c:112-153)
+//   6 k_link_dense   dense cells, one workgroup (256 particles or more) or
+//                    one wave each: run pairs with early exit
 //   7 union-find finish: flatten, group minimum index, sizes
 //   8 labels, compaction of kept groups, stable radix sort by label
 //   9 ring in ascending index order, halo list, FP64 halo sums
@@ -297,93 +300,229 @@ __device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t t
 constexpr uint32_t kBigCell = 256;
 
 // ---------------------------------------------------------------------------
-// 1: keys + histogram
+// 1-3 (dense grid): counting sort by cell without global atomics.
+// Cells are grouped into fine buckets of 2^14 consecutive cells and those into
+// coarse buckets of 256 fine buckets.
+//   A: per tile of kPartTile particles, cell keys and an LDS histogram of
+//      coarse buckets; scan; scatter {key}, {x,y,z,index} by coarse bucket
+//   B: the same inside each coarse bucket, by fine bucket (256 ways)
+//   C: one workgroup per fine bucket: LDS counters for its 2^14 cells,
+//      scan -> cell starts and dense-cell lists, LDS cursors -> cell order
+// 256-way partitions keep a tile's open output lines few enough to be
+// combined in L2; all counting is in LDS.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_keys(const uint8_t *__restrict__ base, uint32_t stride,
-                                              uint32_t offpos, uint32_t N,
-                                              const VolDev *__restrict__ vols,
-                                              const uint32_t *__restrict__ vbeg, int nvol,
-                                              uint32_t *__restrict__ key,
-                                              uint32_t *__restrict__ cnt, uint32_t T,
-                                              uint32_t *__restrict__ dense,
-                                              uint32_t *__restrict__ ndense,
-                                              uint32_t *__restrict__ err) {
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N;
-       p += gridDim.x * blockDim.x) {
-    const int v = vol_of(vbeg, nvol, p);
-    const VolDev &V = vols[v];
-    const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
-    const float x = pp[0], y = pp[1], z = pp[2];
-    // src_v2/read_data.c:810-812 bounds; non-finite values are rejected too.
-    const bool ok = ((double)x >= V.rmin_buff[0]) && ((double)x <= V.rmax_buff[0]) &&
-                    ((double)y >= V.rmin_buff[1]) && ((double)y <= V.rmax_buff[1]) &&
-                    ((double)z >= V.rmin_buff[2]) && ((double)z <= V.rmax_buff[2]);
-    if (!ok) atomicOr(err, 1u);
-    double u;
-    const int ix = ok ? cell1((double)x, V.rmin_buff[0], V.lcell[0], V.nx, &u) : 0;
-    const int iy = ok ? cell1((double)y, V.rmin_buff[1], V.lcell[1], V.ny, &u) : 0;
-    const int iz = ok ? cell1((double)z, V.rmin_buff[2], V.lcell[2], V.nz, &u) : 0;
-    const uint32_t c = cell_id(V, ix, iy, iz);
+constexpr uint32_t kPartTile = 32768;
+constexpr int kFineBits = 14;
+constexpr uint32_t kFineCells = 1u << kFineBits;
+constexpr int kMaxCoarse = 512;  // C < 2^31 cells
+
+__device__ __forceinline__ uint32_t cell_of_record(const uint8_t *__restrict__ base,
+                                                   uint32_t stride, uint32_t offpos, uint32_t p,
+                                                   const VolDev &V, uint32_t *__restrict__ err) {
+  const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
+  const float x = pp[0], y = pp[1], z = pp[2];
+  // src_v2/read_data.c:810-812 bounds; non-finite values are rejected too.
+  const bool ok = ((double)x >= V.rmin_buff[0]) && ((double)x <= V.rmax_buff[0]) &&
+                  ((double)y >= V.rmin_buff[1]) && ((double)y <= V.rmax_buff[1]) &&
+                  ((double)z >= V.rmin_buff[2]) && ((double)z <= V.rmax_buff[2]);
+  if (!ok) atomicOr(err, 1u);
+  double u;
+  const int ix = ok ? cell1((double)x, V.rmin_buff[0], V.lcell[0], V.nx, &u) : 0;
+  const int iy = ok ? cell1((double)y, V.rmin_buff[1], V.lcell[1], V.ny, &u) : 0;
+  const int iz = ok ? cell1((double)z, V.rmin_buff[2], V.lcell[2], V.nz, &u) : 0;
+  return cell_id(V, ix, iy, iz);
+}
+
+// A, count: keys, validation, hist[b * ntiles + tile] (bucket-major, so one
+// exclusive scan gives each (bucket, tile) its first output slot).
+__global__ __launch_bounds__(256) void k_pa_count(const uint8_t *__restrict__ base,
+                                                  uint32_t stride, uint32_t offpos, uint32_t N,
+                                                  const VolDev *__restrict__ vols,
+                                                  const uint32_t *__restrict__ vbeg, int nvol,
+                                                  uint32_t Bc, uint32_t ntiles,
+                                                  uint32_t *__restrict__ key,
+                                                  uint32_t *__restrict__ hist,
+                                                  uint32_t *__restrict__ err) {
+  __shared__ uint32_t h[kMaxCoarse];
+  const uint32_t tile = blockIdx.x;
+  for (uint32_t b = threadIdx.x; b < Bc; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const uint32_t p0 = tile * kPartTile, p1 = min(N, p0 + kPartTile);
+  for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    const uint32_t c = cell_of_record(base, stride, offpos, p, vols[vol_of(vbeg, nvol, p)], err);
     key[p] = c;
-    atomicAdd(cnt + c, 1u);  // result unused: a non-returning atomic
+    atomicAdd(&h[c >> (kFineBits + 8)], 1u);
   }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < Bc; b += blockDim.x) hist[(uint64_t)b * ntiles + tile] = h[b];
 }
 
-// Dense-cell list from the cell counts (one coalesced pass over the cells;
-// wave-aggregated appends).
-__global__ __launch_bounds__(256) void k_dense_list(uint32_t C, const uint32_t *__restrict__ cnt,
-                                                    const VolDev *__restrict__ vols, int nvol,
-                                                    uint32_t T, uint32_t *__restrict__ dense,
-                                                    uint32_t *__restrict__ ndense,
-                                                    uint32_t *__restrict__ dbig,
-                                                    uint32_t *__restrict__ nbig) {
-  const int lane = threadIdx.x & 63;
-  for (uint32_t c0 = (blockIdx.x * blockDim.x) & ~63u; c0 < C; c0 += gridDim.x * blockDim.x) {
-    const uint32_t c = c0 + lane + (threadIdx.x & ~63u);
-    bool is = false;
-    const uint32_t n = (c < C) ? cnt[c] : 0u;
-    if (n >= T) {
-      // volume of cell c (cells are laid out volume after volume)
-      int lo = 0, hi = nvol - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (vols[mid].cbeg <= c) lo = mid;
-        else hi = mid - 1;
-      }
-      is = vols[lo].m[0] > 0;
-    }
-    // two lists: cells below kBigCell particles (one wave each) and the rest
-    const bool big = is && n >= kBigCell;
-    const bool sml = is && !big;
-    const unsigned long long b = __ballot(sml), bb = __ballot(big);
-    if (b) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(ndense, (uint32_t)__popcll(b));
-      base = __shfl(base, 0, 64);
-      if (sml) dense[base + __popcll(b & ((1ull << lane) - 1))] = c;
-    }
-    if (bb) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(nbig, (uint32_t)__popcll(bb));
-      base = __shfl(base, 0, 64);
-      if (big) dbig[base + __popcll(bb & ((1ull << lane) - 1))] = c;
-    }
-  }
-}
-
-// 3: counting-sort scatter.  cnt[c] counts down to 0.
-__global__ __launch_bounds__(256) void k_scatter(const uint8_t *__restrict__ base,
-                                                 uint32_t stride, uint32_t offpos, uint32_t N,
-                                                 const uint32_t *__restrict__ key,
-                                                 const uint32_t *__restrict__ start,
-                                                 uint32_t *__restrict__ cnt,
-                                                 float4 *__restrict__ pos4) {
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N;
-       p += gridDim.x * blockDim.x) {
-    const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
+// A, scatter: keys and {x,y,z,index} to the tile's slots of each coarse bucket.
+__global__ __launch_bounds__(256) void k_pa_scatter(const uint8_t *__restrict__ base,
+                                                    uint32_t stride, uint32_t offpos, uint32_t N,
+                                                    uint32_t Bc, uint32_t ntiles,
+                                                    const uint32_t *__restrict__ key,
+                                                    const uint32_t *__restrict__ off,
+                                                    uint32_t *__restrict__ okey,
+                                                    float4 *__restrict__ opos) {
+  __shared__ uint32_t cur[kMaxCoarse];
+  const uint32_t tile = blockIdx.x;
+  for (uint32_t b = threadIdx.x; b < Bc; b += blockDim.x) cur[b] = off[(uint64_t)b * ntiles + tile];
+  __syncthreads();
+  const uint32_t p0 = tile * kPartTile, p1 = min(N, p0 + kPartTile);
+  for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const uint32_t c = key[p];
-    const uint32_t slot = start[c] + atomicSub(cnt + c, 1u) - 1u;
-    pos4[slot] = make_float4(pp[0], pp[1], pp[2], __uint_as_float(p));
+    const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
+    const uint32_t slot = atomicAdd(&cur[c >> (kFineBits + 8)], 1u);
+    okey[slot] = c;
+    opos[slot] = make_float4(pp[0], pp[1], pp[2], __uint_as_float(p));
+  }
+}
+
+// Coarse bucket starts (Bc + 1 entries) out of the scanned A histogram.
+__global__ void k_pa_starts(uint32_t Bc, uint32_t ntiles, const uint32_t *__restrict__ off,
+                            uint32_t *__restrict__ cstart) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= Bc; b += gridDim.x * blockDim.x)
+    cstart[b] = off[(uint64_t)b * ntiles];
+}
+
+// B: tiles inside coarse buckets.  tinfo[t] = coarse bucket of tile t;
+// t0[c] = first tile of bucket c (t0[Bc] = number of tiles).
+__device__ __forceinline__ void pb_tile(uint32_t t, const uint32_t *__restrict__ tinfo,
+                                        const uint32_t *__restrict__ t0,
+                                        const uint32_t *__restrict__ cstart, uint32_t &c,
+                                        uint32_t &i0, uint32_t &i1, uint32_t &nt, uint32_t &k) {
+  c = tinfo[t];
+  k = t - t0[c];
+  nt = t0[c + 1] - t0[c];
+  i0 = cstart[c] + k * kPartTile;
+  i1 = min(cstart[c + 1], i0 + kPartTile);
+}
+
+__global__ __launch_bounds__(256) void k_pb_count(const uint32_t *__restrict__ tinfo,
+                                                  const uint32_t *__restrict__ t0,
+                                                  const uint32_t *__restrict__ cstart,
+                                                  const uint32_t *__restrict__ ikey,
+                                                  uint32_t *__restrict__ hist) {
+  __shared__ uint32_t h[256];
+  uint32_t c, i0, i1, nt, k;
+  pb_tile(blockIdx.x, tinfo, t0, cstart, c, i0, i1, nt, k);
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+    atomicAdd(&h[(ikey[i] >> kFineBits) & 255u], 1u);
+  __syncthreads();
+  hist[256ull * t0[c] + (uint64_t)threadIdx.x * nt + k] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_pb_scatter(const uint32_t *__restrict__ tinfo,
+                                                    const uint32_t *__restrict__ t0,
+                                                    const uint32_t *__restrict__ cstart,
+                                                    const uint32_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ ikey,
+                                                    const float4 *__restrict__ ipos,
+                                                    uint32_t *__restrict__ okey,
+                                                    float4 *__restrict__ opos) {
+  __shared__ uint32_t cur[256];
+  uint32_t c, i0, i1, nt, k;
+  pb_tile(blockIdx.x, tinfo, t0, cstart, c, i0, i1, nt, k);
+  cur[threadIdx.x] = off[256ull * t0[c] + (uint64_t)threadIdx.x * nt + k];
+  __syncthreads();
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const uint32_t key = ikey[i];
+    const uint32_t slot = atomicAdd(&cur[(key >> kFineBits) & 255u], 1u);
+    okey[slot] = key;
+    opos[slot] = ipos[i];
+  }
+}
+
+// Fine bucket starts (F + 1 entries).
+__global__ void k_pb_starts(uint32_t F, const uint32_t *__restrict__ t0,
+                            const uint32_t *__restrict__ cstart, const uint32_t *__restrict__ off,
+                            uint32_t N, uint32_t *__restrict__ fstart) {
+  for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f <= F; f += gridDim.x * blockDim.x) {
+    if (f == F) {
+      fstart[f] = N;
+      continue;
+    }
+    const uint32_t c = f >> 8, sub = f & 255u, nt = t0[c + 1] - t0[c];
+    fstart[f] = nt ? off[256ull * t0[c] + (uint64_t)sub * nt] : cstart[c];
+  }
+}
+
+__device__ __forceinline__ int vol_of_cell(const VolDev *__restrict__ vols, int nvol, uint32_t c) {
+  int lo = 0, hi = nvol - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (vols[mid].cbeg <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// C: one workgroup per fine bucket, all counting in LDS.
+__global__ __launch_bounds__(256) void k_pc_sort(uint64_t C, const uint32_t *__restrict__ fstart,
+                                                 const uint32_t *__restrict__ ikey,
+                                                 const float4 *__restrict__ ipos,
+                                                 uint32_t *__restrict__ start,
+                                                 float4 *__restrict__ pos4,
+                                                 const VolDev *__restrict__ vols, int nvol,
+                                                 uint32_t T, uint32_t *__restrict__ dense,
+                                                 uint32_t *__restrict__ ndense,
+                                                 uint32_t *__restrict__ dbig,
+                                                 uint32_t *__restrict__ nbig) {
+  __shared__ uint32_t lc[kFineCells + 1];
+  __shared__ uint32_t ws[4];
+  const uint32_t f = blockIdx.x;
+  const uint32_t q0 = fstart[f], q1 = fstart[f + 1];
+  const uint64_t c0 = (uint64_t)f << kFineBits;
+  const uint32_t nc = (uint32_t)min((uint64_t)kFineCells, C - c0);
+  for (uint32_t e = threadIdx.x; e < kFineCells; e += blockDim.x) lc[e] = 0;
+  __syncthreads();
+  for (uint32_t i = q0 + threadIdx.x; i < q1; i += blockDim.x)
+    atomicAdd(&lc[ikey[i] & (kFineCells - 1)], 1u);
+  __syncthreads();
+  // exclusive scan of the 2^14 counters: 64 consecutive per thread
+  constexpr uint32_t kPer = kFineCells / 256;
+  const uint32_t e0 = threadIdx.x * kPer;
+  uint32_t loc = 0;
+  for (uint32_t q = 0; q < kPer; q++) loc += lc[e0 + q];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = loc;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) ws[w] = inc;
+  __syncthreads();
+  uint32_t pre = inc - loc;
+  for (int q = 0; q < w; q++) pre += ws[q];
+  for (uint32_t q = 0; q < kPer; q++) {
+    const uint32_t v = lc[e0 + q];
+    lc[e0 + q] = pre;
+    pre += v;
+  }
+  if (threadIdx.x == 255) lc[kFineCells] = pre;  // bucket total
+  __syncthreads();
+  // cell starts (coalesced) and the dense-cell lists
+  for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) {
+    const uint32_t s = lc[e], n = lc[e + 1] - s;
+    start[c0 + e] = q0 + s;
+    if (n >= T) {
+      const uint32_t cc = (uint32_t)(c0 + e);
+      if (vols[vol_of_cell(vols, nvol, cc)].m[0] > 0) {
+        if (n >= kBigCell) dbig[atomicAdd(nbig, 1u)] = cc;
+        else dense[atomicAdd(ndense, 1u)] = cc;
+      }
+    }
+  }
+  if (c0 + nc == C && threadIdx.x == 0) start[C] = q1;
+  __syncthreads();
+  // scatter with LDS cursors
+  for (uint32_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
+    const uint32_t slot = q0 + atomicAdd(&lc[ikey[i] & (kFineCells - 1)], 1u);
+    pos4[slot] = ipos[i];
   }
 }
 
@@ -796,27 +935,8 @@ __device__ __forceinline__ bool runs_link(const float4 *__restrict__ pos4, uint3
 }
 
 // Run pairs with more than kBigPair particle pairs are queued in LDS and
-// tested by a whole wave: 64 particle pairs per step, ballot exit.
+// tested by 16-lane groups: 16 particle pairs per step, ballot exit.
 constexpr uint32_t kBigPair = 24;
-
-__device__ __forceinline__ bool runs_link_wave(const float4 *__restrict__ pos4, uint32_t ha,
-                                               uint32_t sa, uint32_t hb, uint32_t sb,
-                                               const Rule &R, uint32_t &tests,
-                                               uint32_t &rechecks) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t total = sa * sb;
-  for (uint32_t base = 0; base < total; base += 64) {
-    const uint32_t idx = base + lane;
-    bool l = false;
-    if (idx < total) {
-      const uint32_t i = idx / sb, j = idx - i * sb;
-      l = pair_linked(pos4[ha + i], pos4[hb + j], R, rechecks);
-      tests++;
-    }
-    if (__ballot(l)) return true;
-  }
-  return false;
-}
 
 // Home-cell sub-cell coordinates (0..11) packed 4 bits per axis.
 __device__ __forceinline__ uint16_t pack_home(int x, int y, int z) {
@@ -936,25 +1056,43 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       }
       if (runs_link(pos4, ha, ea, hb, eb, R, tests, rechecks)) lunion(lp, sa, sb);
     };
-    // the queued pairs, one wave each: skip when joined in LDS or under the
-    // same global root now, else a wave-wide test
+    // the queued pairs, four per wave (16-lane groups): skip when joined in
+    // LDS, join when under the same global root now, else a group-wide test
     auto drain = [&]() {
       __syncthreads();
       const uint32_t nq = min(qn, (uint32_t)BIGQ);
-      for (uint32_t k = threadIdx.x >> 6; k < nq; k += blockDim.x >> 6) {
-        const uint32_t e = bigq[k];
-        const uint32_t sa = e >> 16, sb = e & 0xFFFFu;
-        const uint32_t ha = ahead[sa], hb = sb < (uint32_t)na ? ahead[sb] : bhead[sb - na];
-        uint32_t v = 0;
-        if (lane == 0) {
-          v = (lfind(lp, sa) == lfind(lp, sb)) ? 2u : 0u;
-          if (!v && uf_find(par, ha) == uf_find(par, hb)) v = 1u;
+      const uint32_t grp = lane >> 4, gl = lane & 15;
+      for (uint32_t k0 = (threadIdx.x >> 6) * 4; k0 < nq; k0 += (blockDim.x >> 6) * 4) {
+        const uint32_t k = k0 + grp;
+        uint32_t sa = 0, sb = 0, ha = 0, hb = 0;
+        if (k < nq) {
+          const uint32_t e = bigq[k];
+          sa = e >> 16, sb = e & 0xFFFFu;
+          ha = ahead[sa], hb = sb < (uint32_t)na ? ahead[sb] : bhead[sb - na];
         }
-        v = __shfl(v, 0, 64);
-        if (v == 2u) continue;
-        const bool link = (v == 1u) || runs_link_wave(pos4, ha, run_end[ha] - ha, hb,
-                                                      run_end[hb] - hb, R, tests, rechecks);
-        if (link && lane == 0) lunion(lp, sa, sb);
+        uint32_t v = 2u;  // 2: nothing to do, 1: join, 0: test
+        if (k < nq && gl == 0) {
+          if (lfind(lp, sa) != lfind(lp, sb)) v = (uf_find(par, ha) == uf_find(par, hb)) ? 1u : 0u;
+        }
+        v = __shfl(v, lane & ~15u, 64);
+        const uint32_t sza = v == 0u ? run_end[ha] - ha : 0u;
+        const uint32_t szb = v == 0u ? run_end[hb] - hb : 1u;
+        const uint32_t total = sza * szb;
+        bool linked = (v == 1u), done = (v != 0u);
+        for (uint32_t base = 0;; base += 16) {
+          bool l = false;
+          const uint32_t idx = base + gl;
+          if (!done && idx < total) {
+            const uint32_t i = idx / szb, j = idx - i * szb;
+            l = pair_linked(pos4[ha + i], pos4[hb + j], R, rechecks);
+            tests++;
+          }
+          const uint32_t gbits = (uint32_t)(__ballot(l) >> (lane & ~15u)) & 0xFFFFu;
+          if (!done && gbits) linked = true, done = true;
+          if (base + 16 >= total) done = true;
+          if (!__ballot(!done)) break;
+        }
+        if (linked && gl == 0) lunion(lp, sa, sb);
       }
       __syncthreads();
     };
@@ -1106,19 +1244,40 @@ __global__ __launch_bounds__(256) void k_flatten(uint32_t N, uint32_t *__restric
                                                  uint32_t *__restrict__ gsize,
                                                  unsigned long long *__restrict__ ctr) {
   uint32_t roots = 0;
-  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
-       x += gridDim.x * blockDim.x) {
-    uint32_t r = x, p = par[x];
-    while (p != r) {
-      r = p;
-      p = par[r];
+  const uint32_t lane = threadIdx.x & 63;
+  // wave-uniform loop: lanes of a wave reduce together
+  for (uint32_t xb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); xb < N;
+       xb += gridDim.x * blockDim.x) {
+    const uint32_t x = xb + lane;
+    uint32_t r = x;
+    bool member = false;
+    if (x < N) {
+      uint32_t p = par[x];
+      while (p != r) {
+        r = p;
+        p = par[r];
+      }
+      member = (r != x);
+      if (member) par[x] = r;
+      else roots++;
     }
-    if (r != x) {
-      par[x] = r;
-      atomicMin(minorig + r, __float_as_uint(pos4[x].w));
-      atomicAdd(gsize + r, 1u);
-    } else {
-      roots++;
+    const uint32_t o = member ? __float_as_uint(pos4[x].w) : 0xFFFFFFFFu;
+    // members of one group are mostly neighbours in the cell order: one
+    // atomicMin / atomicAdd per distinct root in the wave
+    unsigned long long pend = __ballot(member);
+    while (pend) {
+      const int leader = __ffsll((long long)pend) - 1;
+      const uint32_t rl = __shfl(r, leader, 64);
+      const bool mine = member && r == rl;
+      const unsigned long long m = __ballot(mine);
+      uint32_t v = mine ? o : 0xFFFFFFFFu;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, 64));
+      if ((int)lane == leader) {
+        atomicMin(minorig + rl, v);
+        atomicAdd(gsize + rl, (uint32_t)__popcll(m));
+      }
+      pend &= ~m;
     }
   }
   uint64_t t = roots;
@@ -1462,7 +1621,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
       if (C >= 0x7FFFFFF0ull) return FOF_E_RANGE;
     }
   }
-  const uint64_t Csz = sparse ? N : C;  // entries of the count and start tables
+  const uint64_t Csz = sparse ? N : C;  // entries of the start table
 
   // Variable linking-length tables are copied to the device back to back.
   size_t vt_entries = 0;
@@ -1529,7 +1688,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   int rc = 0;
   if ((rc = ensure(W.pos4, N * 16))) return rc;
   if ((rc = ensure(W.key, N * 4 + 64))) return rc;
-  if ((rc = ensure(W.cnt, Csz * 4 + 64))) return rc;
+  if ((rc = ensure(W.cnt, (sparse ? N : 1) * 4 + 64))) return rc;  // occupied-cell flags
   if ((rc = ensure(W.start, (Csz + 2) * 4 + 64))) return rc;
   if ((rc = ensure(W.par, N * 4 + 64))) return rc;
   if ((rc = ensure(W.runend, N * 4 + 64))) return rc;
@@ -1540,7 +1699,13 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   const uint64_t q4 = (N + 63) / 64 * 64;  // u32 entries per sort array
   if ((rc = ensure(W.sortbuf, 16 * q4 + 256))) return rc;
   if ((rc = ensure(W.dense, 2 * ndense_cap * 4 + 64))) return rc;  // two lists
-  size_t tmpb = std::max(std::max(scan_tmp_bytes(Csz + 1), sort_tmp_bytes(N)), compact_tmp_bytes(N));
+  size_t tmpb = std::max(sort_tmp_bytes(N), compact_tmp_bytes(N));
+  {
+    // the partition scans: coarse buckets x tiles, 256 x tiles of the fine pass
+    const uint64_t F = (Csz + kFineCells - 1) >> kFineBits, Bc = (F + 255) >> 8;
+    const uint64_t ntA = (N + kPartTile - 1) / kPartTile;
+    tmpb = std::max(tmpb, scan_tmp_bytes(std::max(Bc * ntA, 256 * (ntA + Bc)) + 1));
+  }
   if ((rc = ensure(W.tmp, tmpb))) return rc;
   if ((rc = ensure(W.vols, nvol * sizeof(VolDev) + vt_entries * sizeof(double) + 256))) return rc;
   if ((rc = ensure(W.vbeg, (nvol + 1) * 4 + 64))) return rc;
@@ -1648,28 +1813,73 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   uint32_t hsmall[8];
   uint32_t ndense = 0, nbig = 0;  // dense cells below / from kBigCell particles
   if (!sparse) {
-    // 1-3: keys + histogram, scan, counting-sort scatter
-    FOF_HIP(hipMemsetAsync(cnt, 0, C * 4, st));
+    // 1-3: counting sort by cell: partitions A (coarse) and B (fine), then C
+    const uint32_t F = (uint32_t)((C + kFineCells - 1) >> kFineBits);  // fine buckets
+    const uint32_t Bc = (F + 255) >> 8;                                 // coarse buckets
+    if (Bc > (uint32_t)kMaxCoarse) return FOF_E_RANGE;
+    const uint32_t ntA = (uint32_t)((N + kPartTile - 1) / kPartTile);
+    const uint32_t ntBmax = ntA + Bc;
+    const uint64_t nA = (uint64_t)Bc * ntA, nB = 256ull * ntBmax;
+    if ((rc = ensure(W.part, (2 * nA + 2 * nB + 2 * (uint64_t)Bc + ntBmax + F + 512) * 4)))
+      return rc;
+    uint32_t *histA = static_cast<uint32_t *>(W.part.ptr);
+    uint32_t *offA = histA + nA + 32;      // nA + 1
+    uint32_t *histB = offA + nA + 32;      // nB
+    uint32_t *offB = histB + nB + 32;      // nB + 1
+    uint32_t *cst = offB + nB + 32;        // Bc + 1
+    uint32_t *t0 = cst + Bc + 32;          // Bc + 1
+    uint32_t *tinfo = t0 + Bc + 32;        // ntBmax
+    uint32_t *fst = tinfo + ntBmax + 32;   // F + 1
+    uint32_t *k1 = gsize, *k2 = label;     // free until the union-find finish / k_label
+    float4 *p1 = pos4;                     // pass A output, consumed before pass C writes
+    float4 *p2 = reinterpret_cast<float4 *>(sk0);
     if ((rc = tb(0))) return rc;
-    k_keys<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, key, cnt, T,
-                               dense, small + 1, small + 0);
-    FOF_HIP(hipGetLastError());
-    k_dense_list<<<grid_for(C), 256, 0, st>>>((uint32_t)C, cnt, dV, (int)nvol, T, dense, small + 1,
-                                              dbig, small + 7);
+    k_pa_count<<<ntA, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, Bc, ntA, key,
+                                    histA, small + 0);
     FOF_HIP(hipGetLastError());
     if ((rc = te(0))) return rc;
+    if ((rc = tb(1))) return rc;
+    if ((rc = scan_u32(histA, offA, nA, W.tmp.ptr, st))) return rc;
+    k_pa_starts<<<grid_for(Bc + 1), 256, 0, st>>>(Bc, ntA, offA, cst);
+    FOF_HIP(hipGetLastError());
+    std::vector<uint32_t> hc(Bc + 1), ht0(Bc + 1);
+    FOF_HIP(hipMemcpyAsync(hc.data(), cst, (Bc + 1) * 4, hipMemcpyDeviceToHost, st));
     FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
     FOF_HIP(hipStreamSynchronize(st));
     if (hsmall[0]) return FOF_E_DOMAIN;
-    ndense = hsmall[1];
-    nbig = hsmall[7];
-    if ((rc = tb(1))) return rc;
-    if ((rc = scan_u32(cnt, start, C, W.tmp.ptr, st))) return rc;
+    std::vector<uint32_t> hti;
+    hti.reserve(ntBmax);
+    for (uint32_t c = 0; c < Bc; c++) {
+      ht0[c] = (uint32_t)hti.size();
+      const uint32_t nt = (hc[c + 1] - hc[c] + kPartTile - 1) / kPartTile;
+      for (uint32_t k = 0; k < nt; k++) hti.push_back(c);
+    }
+    ht0[Bc] = (uint32_t)hti.size();
+    const uint32_t ntB = ht0[Bc];
+    FOF_HIP(hipMemcpyAsync(t0, ht0.data(), (Bc + 1) * 4, hipMemcpyHostToDevice, st));
+    if (ntB) FOF_HIP(hipMemcpyAsync(tinfo, hti.data(), ntB * 4, hipMemcpyHostToDevice, st));
+    FOF_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
     if ((rc = te(1))) return rc;
     if ((rc = tb(2))) return rc;
-    k_scatter<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, key, start, cnt, pos4);
+    k_pa_scatter<<<ntA, 256, 0, st>>>(dbase, stride, offpos, NN, Bc, ntA, key, offA, k1, p1);
+    FOF_HIP(hipGetLastError());
+    if (ntB) {
+      k_pb_count<<<ntB, 256, 0, st>>>(tinfo, t0, cst, k1, histB);
+      FOF_HIP(hipGetLastError());
+      if ((rc = scan_u32(histB, offB, 256ull * ntB, W.tmp.ptr, st))) return rc;
+      k_pb_scatter<<<ntB, 256, 0, st>>>(tinfo, t0, cst, offB, k1, p1, k2, p2);
+      FOF_HIP(hipGetLastError());
+    }
+    k_pb_starts<<<grid_for(F + 1), 256, 0, st>>>(F, t0, cst, offB, NN, fst);
+    FOF_HIP(hipGetLastError());
+    k_pc_sort<<<F, 256, 0, st>>>((uint64_t)C, fst, k2, p2, start, pos4, dV, (int)nvol, T, dense,
+                                 small + 1, dbig, small + 7);
     FOF_HIP(hipGetLastError());
     if ((rc = te(2))) return rc;
+    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
+    FOF_HIP(hipStreamSynchronize(st));
+    ndense = hsmall[1];
+    nbig = hsmall[7];
   } else {
     // 1-3: composite keys, LSD sort by (volume, cell), occupied cells, hash
     if ((rc = tb(0))) return rc;
@@ -1873,7 +2083,7 @@ extern "C" void fof_engine_free(fof_ctx *ctx) {
   Scratch *all[] = {&S.w.pos4, &S.w.key, &S.w.cnt, &S.w.start, &S.w.par, &S.w.runend,
                     &S.w.gsize, &S.w.label, &S.w.sortbuf,
                     &S.w.dense, &S.w.tmp, &S.w.vols, &S.w.vbeg, &S.w.small, &S.w.input,
-                    &S.w.out_next, &S.w.out_label, &S.w.out_halo, &S.w.hhead, &S.w.hoff,
+                    &S.w.out_next, &S.w.out_label, &S.w.out_halo, &S.w.hhead, &S.w.hoff, &S.w.part,
                     &S.w.hash, &S.w.fine, &S.w.rtab};
   for (Scratch *s : all) release(*s);
   if (S.events)

@@ -73,7 +73,7 @@ int compact_flagged(const uint32_t *src, uint64_t n, uint32_t flag, uint32_t *ou
 // Device workspace owned by a context (grow-only).
 struct Work {
   Scratch pos4, key, cnt, start, par, runend, gsize, label, sortbuf, dense, tmp, vols, vbeg,
-      small, input, out_next, out_label, out_halo, hhead, hoff, hash, fine, rtab;
+      small, input, out_next, out_label, out_halo, hhead, hoff, hash, fine, rtab, part;
 };
 
 struct EngineState {

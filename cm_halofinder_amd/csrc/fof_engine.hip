@@ -869,9 +869,9 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
 // neighbours) or particles (sparse neighbours) are taken a chunk at a time.
 constexpr int kMaxRuns = 1728;
 constexpr uint32_t kMaxT = 512;
-// Default dense threshold: the sweep over clumpy 1024^3 boxes (b = 0.2,
-// about one particle per cell) is flat within 5% from 64 to 256.
-constexpr uint32_t kDefaultT = 96;
+// Default dense threshold: on clumpy 1024^3 boxes (b = 0.2, about one
+// particle per cell) the call time is flat within 2% from 32 to 96.
+constexpr uint32_t kDefaultT = 48;
 
 __device__ __forceinline__ uint32_t pack_rel(int a, int b, int c) {
   return ((uint32_t)(a & 255)) | ((uint32_t)(b & 255) << 8) | ((uint32_t)(c & 255) << 16);
@@ -943,7 +943,12 @@ __device__ __forceinline__ uint16_t pack_home(int x, int y, int z) {
   return (uint16_t)(x | (y << 4) | (z << 8));
 }
 
-template <int NT, int MAXRUNS, int MAXRUNSB, int BIGQ>
+// Sub-cell offsets within reach are listed (packed 8 bits per axis) when
+// there are at most MAXOFF of them (5^3 - 1 = 124 at the usual reach of 2);
+// otherwise run pairs are enumerated directly.
+__device__ __forceinline__ uint32_t pack_off(int a, int b, int c) { return pack_rel(a, b, c); }
+
+template <int NT, int MAXRUNS, int MAXRUNSB, int BIGQ, int MAXOFF>
 __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ dense,
                                                     const uint32_t *__restrict__ ndense,
                                                     const float4 *__restrict__ pos4,
@@ -965,10 +970,11 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
   __shared__ uint32_t lp[MAXRUNS + MAXRUNSB];
   __shared__ uint16_t amap[kMaxRuns];  // sub-cell of A -> run slot (<= 12^3 sub-cells)
   __shared__ uint32_t bigq[BIGQ];
+  __shared__ uint32_t offs[MAXOFF];     // offsets within reach: touching ones first
   __shared__ uint32_t nbS[13], nbOff[14], nbTab[13];
   __shared__ int nbLo[13][3], nbHi[13][3];
   __shared__ uint8_t nbDense[13];
-  __shared__ uint32_t cnt, qn;
+  __shared__ uint32_t cnt, qn, noff, nfwd;
   uint32_t tests = 0, rechecks = 0;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nd = *ndense;
@@ -985,6 +991,14 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
     int reach[3];
     for (int a = 0; a < 3; a++)  // largest sub-cell offset whose gap can be within reach
       reach[a] = min((int)floor(sqrt(R.prune2 / V.s2[a])) + 1, mm[a]);
+    // Sub-cell gap pruning in FP32: the prune radius carries a 1e-4 margin,
+    // far above the FP32 rounding of the gap.
+    const float gs0 = (float)V.s2[0], gs1 = (float)V.s2[1], gs2 = (float)V.s2[2];
+    const float gp2 = (float)R.prune2;
+    auto gap_ok = [&](int dx, int dy, int dz) {
+      const int gx = max(abs(dx) - 1, 0), gy = max(abs(dy) - 1, 0), gz = max(abs(dz) - 1, 0);
+      return (float)(gx * gx) * gs0 + (float)(gy * gy) * gs1 + (float)(gz * gz) * gs2 <= gp2;
+    };
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = 0xFFFF;
     // neighbour table: the 13 forward cells of src_v2/FOF.c:122-153, their
     // items (runs of dense neighbours, particles of sparse ones) and the
@@ -1013,11 +1027,28 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
         nbLo[k][a] = lo, nbHi[k][a] = hi;
       }
     }
-    if (threadIdx.x == 0) cnt = 0, qn = 0;
+    if (threadIdx.x == 0) cnt = 0, qn = 0, noff = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
       nbOff[0] = 0;
       for (int k = 0; k < 13; k++) nbOff[k + 1] += nbOff[k];
+    }
+    // offsets within reach (gap test passed), in two passes so that the
+    // touching ones (|d| <= 1 on every axis) come first; forward ones (the
+    // lexicographically positive half) are counted in nfwd
+    const int wx = 2 * reach[0] + 1, wy = 2 * reach[1] + 1, wz = 2 * reach[2] + 1;
+    const int wall = wx * wy * wz;
+    const bool use_off = wall <= MAXOFF + 1;  // the zero offset is not listed
+    if (use_off) {
+      for (int pass = 0; pass < 2; pass++) {
+        for (int t = threadIdx.x; t < wall; t += blockDim.x) {
+          const int dx = t % wx - reach[0], dy = (t / wx) % wy - reach[1], dz = t / (wx * wy) - reach[2];
+          const bool touch = abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1;
+          if ((pass == 0) != touch || (dx == 0 && dy == 0 && dz == 0) || !gap_ok(dx, dy, dz)) continue;
+          offs[atomicAdd(&noff, 1u)] = pack_off(dx, dy, dz);
+        }
+        __syncthreads();
+      }
     }
     // the home cell's runs (one per occupied sub-cell) from its run table
     const uint32_t atab = rtoff[sA];
@@ -1035,15 +1066,8 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
     }
     __syncthreads();
     const int na = (int)cnt;
-    // Sub-cell gap pruning in FP32: the prune radius carries a 1e-4 margin,
-    // far above the FP32 rounding of the gap.
-    const float gs0 = (float)V.s2[0], gs1 = (float)V.s2[1], gs2 = (float)V.s2[2];
-    const float gp2 = (float)R.prune2;
-    auto gap_ok = [&](int dx, int dy, int dz) {
-      const int gx = max(abs(dx) - 1, 0), gy = max(abs(dy) - 1, 0), gz = max(abs(dz) - 1, 0);
-      return (float)(gx * gx) * gs0 + (float)(gy * gy) * gs1 + (float)(gz * gz) * gs2 <= gp2;
-    };
-    // a run pair (LDS slots sa < sb, heads ha, hb) not yet joined: small
+    const uint32_t nof = use_off ? noff : 0u;
+    // a run pair (LDS slots sa, sb with heads ha, hb) not yet joined: small
     // pairs are tested here, big ones go to the wave queue
     auto test_pair = [&](uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb) {
       const uint32_t ea = run_end[ha], eb = run_end[hb];
@@ -1095,52 +1119,56 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
         if (linked && gl == 0) lunion(lp, sa, sb);
       }
       __syncthreads();
+      if (threadIdx.x == 0) qn = 0;
+      __syncthreads();
     };
-    // (A, A): every run pair, or forward sub-cell offsets through the map
-    // when the cell has more runs than the offset window
-    const int wAA = (reach[2] + 1) * (2 * reach[1] + 1) * (2 * reach[0] + 1);
-    if (na <= wAA) {
-      const uint32_t npair = (uint32_t)na * (uint32_t)na;
-      for (uint32_t p = threadIdx.x; p < npair; p += blockDim.x) {
-        const uint32_t ia = p / (uint32_t)na, ib = p - ia * (uint32_t)na;
-        if (ib <= ia) continue;
-        const uint32_t ra = arel[ia], rb = arel[ib];
-        if (!gap_ok((int)(rb & 15) - (int)(ra & 15), (int)((rb >> 4) & 15) - (int)((ra >> 4) & 15),
-                    (int)(rb >> 8) - (int)(ra >> 8)))
-          continue;
-        if (lfind(lp, ia) == lfind(lp, ib)) continue;
-        // joined elsewhere (snapshot of the global roots), or test
-        if (agr[ia] == agr[ib]) lunion(lp, ia, ib);
-        else test_pair(ia, ib, ahead[ia], ahead[ib]);
-      }
-    } else {
-      for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
-        const uint32_t ra = arel[ia];
-        const int ax = ra & 15, ay = (ra >> 4) & 15, az = ra >> 8;
-        for (int dz = 0; dz <= reach[2]; dz++) {
-          const int bz = az + dz;
-          if (bz >= mm[2]) break;
-          for (int dy = -reach[1]; dy <= reach[1]; dy++) {
-            if (dz == 0 && dy < 0) continue;
-            const int by = ay + dy;
-            if (by < 0 || by >= mm[1]) continue;
-            for (int dx = -reach[0]; dx <= reach[0]; dx++) {
-              if (dz == 0 && dy == 0 && dx <= 0) continue;
-              const int bxx = ax + dx;
-              if (bxx < 0 || bxx >= mm[0]) continue;
-              if (!gap_ok(dx, dy, dz)) continue;
-              const uint32_t ib = amap[(bz * mm[1] + by) * mm[0] + bxx];
-              if (ib == 0xFFFF) continue;
-              if (lfind(lp, ia) == lfind(lp, ib)) continue;
-              if (agr[ia] == agr[ib]) lunion(lp, ia, ib);
-              else if (ia < (int)ib) test_pair(ia, ib, ahead[ia], ahead[ib]);
-              else test_pair(ib, ia, ahead[ib], ahead[ia]);
-            }
+    // a candidate pair of slots: skip if joined, join on equal snapshots,
+    // else test
+    auto consider = [&](uint32_t sa, uint32_t sb, uint32_t ga, uint32_t gb, uint32_t ha,
+                        uint32_t hb) {
+      if (lfind(lp, sa) == lfind(lp, sb)) return;
+      if (ga == gb) lunion(lp, sa, sb);
+      else test_pair(sa, sb, ha, hb);
+    };
+    // (A, A) in rounds of at most BIGQ candidates, each followed by a drain,
+    // so that merges found early prune the later rounds: either every run
+    // pair, or (offset-major, touching offsets first) the forward offsets
+    // through the sub-cell map
+    {
+      const bool map = use_off && (uint32_t)na > nof / 2;
+      const uint32_t total = map ? nof * (uint32_t)na : (uint32_t)na * (uint32_t)na;
+      for (uint32_t base = 0; base < total; base += BIGQ) {
+        const uint32_t pe = min(total, base + (uint32_t)BIGQ);
+        for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
+          uint32_t ia, ib;
+          if (map) {
+            const uint32_t oi = p / (uint32_t)na;
+            ia = p - oi * (uint32_t)na;
+            const uint32_t o = offs[oi];
+            const int dx = rel_x(o), dy = rel_y(o), dz = rel_z(o);
+            if (!(dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0))))) continue;  // forward
+            const uint32_t ra = arel[ia];
+            const int bx = (int)(ra & 15) + dx, by = (int)((ra >> 4) & 15) + dy,
+                      bz = (int)(ra >> 8) + dz;
+            if (bx < 0 || bx >= mm[0] || by < 0 || by >= mm[1] || bz >= mm[2]) continue;
+            ib = amap[(bz * mm[1] + by) * mm[0] + bx];
+            if (ib == 0xFFFF) continue;
+          } else {
+            ia = p / (uint32_t)na;
+            ib = p - ia * (uint32_t)na;
+            if (ib <= ia) continue;
+            const uint32_t ra = arel[ia], rb = arel[ib];
+            if (!gap_ok((int)(rb & 15) - (int)(ra & 15),
+                        (int)((rb >> 4) & 15) - (int)((ra >> 4) & 15),
+                        (int)(rb >> 8) - (int)(ra >> 8)))
+              continue;
           }
+          const uint32_t s0 = min(ia, ib), s1 = max(ia, ib);
+          consider(s0, s1, agr[s0], agr[s1], ahead[s0], ahead[s1]);
         }
+        drain();
       }
     }
-    drain();
     // publish the home cell's merges before the neighbour pass
     for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
       const uint32_t r = lfind(lp, ia);
@@ -1148,11 +1176,9 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
     }
     // (A, B): the neighbours' items in chunks of MAXRUNSB
     const uint32_t ntot = nbOff[13];
-    const int wAB = min(2 * reach[0] + 1, mm[0]) * min(2 * reach[1] + 1, mm[1]) *
-                    min(2 * reach[2] + 1, mm[2]);
     for (uint32_t c0 = 0; c0 < ntot; c0 += MAXRUNSB) {
       const uint32_t c1 = min(ntot, c0 + (uint32_t)MAXRUNSB);
-      if (threadIdx.x == 0) cnt = 0, qn = 0;
+      if (threadIdx.x == 0) cnt = 0;
       __syncthreads();
       for (uint32_t g = c0 + threadIdx.x; g < c1; g += blockDim.x) {
         int q = 0;
@@ -1179,46 +1205,40 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
         lp[na + k] = na + k;
       }
       __syncthreads();
-      const int nb = (int)cnt;
-      // each neighbour run against the home runs: every pair, or the home
-      // sub-cells within reach through the map when that window is smaller
-      if (na <= wAB) {
-        const uint32_t npair = (uint32_t)na * (uint32_t)nb;
-        for (uint32_t p = threadIdx.x; p < npair; p += blockDim.x) {
-          const uint32_t ib = p / (uint32_t)na, ia = p - ib * (uint32_t)na;
-          const uint32_t ra = arel[ia], rb = brel[ib];
-          if (!gap_ok(rel_x(rb) - (int)(ra & 15), rel_y(rb) - (int)((ra >> 4) & 15),
-                      rel_z(rb) - (int)(ra >> 8)))
-            continue;
-          if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
-          if (agr[ia] == bgr[ib]) lunion(lp, ia, na + ib);
-          else test_pair(ia, na + ib, ahead[ia], bhead[ib]);
+      const uint32_t nb = cnt;
+      // every (home run, neighbour item) pair within reach, in rounds:
+      // direct, or (offset-major, touching first) through the sub-cell map
+      const bool map = use_off && (uint32_t)na > nof;
+      const uint32_t total = map ? nof * nb : (uint32_t)na * nb;
+      for (uint32_t base = 0; base < total; base += BIGQ) {
+        const uint32_t pe = min(total, base + (uint32_t)BIGQ);
+        for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
+          uint32_t ia, ib;
+          if (map) {
+            const uint32_t oi = p / nb;
+            ib = p - oi * nb;
+            const uint32_t o = offs[oi], rb = brel[ib];
+            const int ax = rel_x(rb) + rel_x(o), ay = rel_y(rb) + rel_y(o),
+                      az = rel_z(rb) + rel_z(o);
+            if (ax < 0 || ax >= mm[0] || ay < 0 || ay >= mm[1] || az < 0 || az >= mm[2]) continue;
+            ia = amap[(az * mm[1] + ay) * mm[0] + ax];
+            if (ia == 0xFFFF) continue;
+          } else {
+            ib = p / (uint32_t)na;
+            ia = p - ib * (uint32_t)na;
+            const uint32_t ra = arel[ia], rb = brel[ib];
+            if (!gap_ok(rel_x(rb) - (int)(ra & 15), rel_y(rb) - (int)((ra >> 4) & 15),
+                        rel_z(rb) - (int)(ra >> 8)))
+              continue;
+          }
+          consider(ia, na + ib, agr[ia], bgr[ib], ahead[ia], bhead[ib]);
         }
-      } else {
-        for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
-          const uint32_t rb = brel[ib];
-          const int bx = rel_x(rb), by = rel_y(rb), bz = rel_z(rb);
-          const uint32_t hb = bhead[ib];
-          const int z0 = max(bz - reach[2], 0), z1 = min(bz + reach[2], mm[2] - 1);
-          const int y0 = max(by - reach[1], 0), y1 = min(by + reach[1], mm[1] - 1);
-          const int x0 = max(bx - reach[0], 0), x1 = min(bx + reach[0], mm[0] - 1);
-          for (int az = z0; az <= z1; az++)
-            for (int ay = y0; ay <= y1; ay++)
-              for (int ax = x0; ax <= x1; ax++) {
-                if (!gap_ok(ax - bx, ay - by, az - bz)) continue;
-                const uint32_t ia = amap[(az * mm[1] + ay) * mm[0] + ax];
-                if (ia == 0xFFFF) continue;
-                if (lfind(lp, ia) == lfind(lp, na + ib)) continue;
-                if (agr[ia] == bgr[ib]) lunion(lp, ia, na + ib);
-                else test_pair(ia, na + ib, ahead[ia], hb);
-              }
-        }
+        drain();
       }
-      drain();
       // commit: a neighbour run joined in LDS joins its (home-cell) root's run
-      for (int ib = threadIdx.x; ib < nb; ib += blockDim.x) {
+      for (uint32_t ib = threadIdx.x; ib < nb; ib += blockDim.x) {
         const uint32_t r = lfind(lp, na + ib);
-        if (r != (uint32_t)(na + ib)) uf_union(par, bhead[ib], ahead[r]);
+        if (r != na + ib) uf_union(par, bhead[ib], ahead[r]);
       }
       __syncthreads();
     }
@@ -1300,13 +1320,29 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
   for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
        x += gridDim.x * blockDim.x) {
     const uint32_t r = par[x];
+    const uint32_t g = gsize[r];
+    if (r == x && g == 0) continue;  // a singleton: k_label_init wrote it
     const uint32_t o = __float_as_uint(pos4[x].w);
     const uint32_t orr = (r == x) ? o : __float_as_uint(pos4[r].w);
-    const uint32_t lab = min(minorig[r], orr), sz = gsize[r] + 1u;
+    const uint32_t lab = min(minorig[r], orr), sz = g + 1u;
     const VolDev &V = vols[vol_of(vbeg, nvol, x)];
     const bool keep = (sz >= 2u) || !((double)sz < V.nphalomin);
     label[o] = lab | (keep ? kFlag : 0u);
     if (o == lab) hsize[lab] = sz;
+  }
+}
+
+// Every particle as a singleton (label = own index, group size 1, kept only
+// when Nphalomin <= 1); k_label overwrites the members of larger groups.
+__global__ __launch_bounds__(256) void k_label_init(uint32_t N, const VolDev *__restrict__ vols,
+                                                    const uint32_t *__restrict__ vbeg, int nvol,
+                                                    uint32_t *__restrict__ label,
+                                                    uint32_t *__restrict__ hsize) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += gridDim.x * blockDim.x) {
+    const bool keep = !(1.0 < vols[vol_of(vbeg, nvol, i)].nphalomin);
+    label[i] = i | (keep ? kFlag : 0u);
+    hsize[i] = 1u;
   }
 }
 
@@ -1953,7 +1989,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if ((rc = te(5))) return rc;
   if (nbig) {
     if ((rc = tb(6))) return rc;
-    k_link_dense<256, kMaxRuns, 512, 768><<<std::min(nbig, 8192u), 256, 0, st>>>(
+    k_link_dense<256, kMaxRuns, 448, 512, 512><<<std::min(nbig, 8192u), 256, 0, st>>>(
         dbig, small + 7, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
         small + 4);
     FOF_HIP(hipGetLastError());
@@ -1961,7 +1997,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   }
   if (ndense) {
     if ((rc = tb(15))) return rc;
-    k_link_dense<64, (int)kBigCell, 128, 152><<<std::min(ndense, 65536u), 64, 0, st>>>(
+    k_link_dense<64, (int)kBigCell, 128, 128, 128><<<std::min(ndense, 65536u), 64, 0, st>>>(
         dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
         small + 4);
     FOF_HIP(hipGetLastError());
@@ -1976,6 +2012,8 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   FOF_HIP(hipGetLastError());
   if ((rc = te(7))) return rc;
   if ((rc = tb(8))) return rc;
+  k_label_init<<<gN, 256, 0, st>>>(NN, dV, dVB, (int)nvol, label, hsize);
+  FOF_HIP(hipGetLastError());
   k_label<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, dV, dVB, (int)nvol, label, hsize);
   FOF_HIP(hipGetLastError());
   k_out_init<<<gN, 256, 0, st>>>(NN, label, dVB, (int)nvol, d_label, d_next);

@@ -778,6 +778,62 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
     uint32_t rx = x;  // x's root, refreshed by every join
     const int nx = V.nx, ny = V.ny, nz = V.nz;
     const int xl = (cp.ix > 0) ? 1 : 0, xr = (cp.ix + 1 < nx) ? 1 : 0;
+    if (!G.sparse && V.vntab == 0 && V.m[0] > 0) {
+      // Constant l on a dense grid: k_link_dense links every pair that has a
+      // particle in a dense cell (its forward cells, and its backward cells
+      // that are sparse).  Here: the sparse cells among the own cell (after
+      // x) and the 13 forward cells, partners loaded eight at a time.
+      // Row segments: own row (ix, ix+1); (iy+1, iz) and (iy-1..iy+1, iz+1)
+      // over ix-1..ix+1.  b[r][0..3] are the segment's cell boundaries.
+      uint32_t b[5][4];
+      bool ok[5];
+      b[0][0] = x + 1, b[0][1] = e, b[0][2] = xr ? G.start[c + 2] : e, b[0][3] = b[0][2];
+      ok[0] = true;
+#pragma unroll
+      for (int r = 1; r < 5; r++) {
+        const int ry = (r == 1) ? cp.iy + 1 : cp.iy + (r - 3);
+        const int rz = (r == 1) ? cp.iz : cp.iz + 1;
+        ok[r] = ry >= 0 && ry < ny && rz < nz;
+        const uint32_t c0 = ok[r] ? cell_id(V, cp.ix - xl, ry, rz) : 0u;
+        const int ncell = xl + 1 + xr;
+#pragma unroll
+        for (int k = 0; k < 4; k++) b[r][k] = ok[r] ? G.start[c0 + min(k, ncell)] : 0u;
+      }
+#pragma unroll
+      for (int r = 0; r < 5; r++) {
+        if (!ok[r]) continue;
+        // a dense cell (own row: only ix+1 can be) is skipped
+        const bool d0 = r > 0 && b[r][1] - b[r][0] >= T;
+        const bool d1 = b[r][2] - b[r][1] >= T;
+        const bool d2 = r > 0 && b[r][3] - b[r][2] >= T;
+        uint32_t j = b[r][0];
+        for (;;) {
+          if (d0 && j < b[r][1]) j = b[r][1];
+          if (d1 && j >= b[r][1] && j < b[r][2]) j = b[r][2];
+          if (d2 && j >= b[r][2]) j = b[r][3];
+          if (j >= b[r][3]) break;
+          uint32_t je = min(j + 8u, b[r][3]);
+          if (d1 && j < b[r][1]) je = min(je, b[r][1]);
+          if (d2 && j < b[r][2]) je = min(je, b[r][2]);
+          const uint32_t n = je - j;
+          float4 q[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) q[k] = ((uint32_t)k < n) ? pos4[j + k] : px;
+          uint32_t hit = 0;
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+            hit |= ((uint32_t)k < n && pair_linked(px, q[k], R, rechecks)) ? (1u << k) : 0u;
+          tests += n;
+          while (hit) {
+            const int k = __builtin_ctz(hit);
+            hit &= hit - 1;
+            rx = uf_join(par, rx, j + k);
+          }
+          j = je;
+        }
+      }
+      continue;
+    }
     // Up to five contiguous ranges cover the own cell (after x) and the 13
     // forward cells: (ix+1,iy,iz); row (iy+1,iz); rows (iy-1..iy+1, iz+1).
     uint32_t r0[5], r1[5];
@@ -971,10 +1027,10 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
   __shared__ uint16_t amap[kMaxRuns];  // sub-cell of A -> run slot (<= 12^3 sub-cells)
   __shared__ uint32_t bigq[BIGQ];
   __shared__ uint32_t offs[MAXOFF];     // offsets within reach: touching ones first
-  __shared__ uint32_t nbS[13], nbOff[14], nbTab[13];
-  __shared__ int nbLo[13][3], nbHi[13][3];
-  __shared__ uint8_t nbDense[13];
-  __shared__ uint32_t cnt, qn, noff, nfwd;
+  __shared__ uint32_t nbS[26], nbOff[27], nbTab[26];
+  __shared__ int nbLo[26][3], nbHi[26][3];
+  __shared__ uint8_t nbDense[26];
+  __shared__ uint32_t cnt, qn, noff;
   uint32_t tests = 0, rechecks = 0;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nd = *ndense;
@@ -1000,21 +1056,26 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       return (float)(gx * gx) * gs0 + (float)(gy * gy) * gs1 + (float)(gz * gz) * gs2 <= gp2;
     };
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = 0xFFFF;
-    // neighbour table: the 13 forward cells of src_v2/FOF.c:122-153, their
-    // items (runs of dense neighbours, particles of sparse ones) and the
-    // window of sub-cells that can reach A
-    if (threadIdx.x < 13) {
-      const int k = threadIdx.x;
+    // neighbour table: the 13 forward cells of src_v2/FOF.c:122-153 and,
+    // for a constant l on a dense grid, the 13 backward cells when sparse
+    // (k_link_sparse leaves their pairs with dense cells to this kernel);
+    // their items (runs of dense neighbours, particles of sparse ones) and
+    // the window of sub-cells that can reach A
+    if (threadIdx.x < 26) {
+      const int k = threadIdx.x, kf = k % 13, sg = k < 13 ? 1 : -1;
       int dd[3] = {0, 0, 0};
-      if (k == 0) dd[0] = 1;
-      else if (k < 4) dd[1] = 1, dd[0] = k - 2;
-      else dd[2] = 1, dd[1] = (k - 4) / 3 - 1, dd[0] = (k - 4) % 3 - 1;
+      if (kf == 0) dd[0] = 1;
+      else if (kf < 4) dd[1] = 1, dd[0] = kf - 2;
+      else dd[2] = 1, dd[1] = (kf - 4) / 3 - 1, dd[0] = (kf - 4) % 3 - 1;
+      dd[0] *= sg, dd[1] *= sg, dd[2] *= sg;
       const int bx = ix + dd[0], by = iy + dd[1], bz = iz + dd[2];
       uint32_t sB = 0, eB = 0;
-      if (!(bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz >= V.nz)) {
+      const bool want = k < 13 || (!G.sparse && V.vntab == 0);
+      if (want && !(bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz < 0 || bz >= V.nz)) {
         const uint32_t cb = cell_index(G, V, bx, by, bz);
         if (cb != kNone) sB = G.start[cb], eB = G.start[cb + 1];
       }
+      if (k >= 13 && eB - sB >= T) eB = sB;  // a dense backward cell links this pair itself
       const bool dn = (eB - sB >= T);
       nbS[k] = sB;
       nbDense[k] = dn ? 1 : 0;
@@ -1031,7 +1092,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
     __syncthreads();
     if (threadIdx.x == 0) {
       nbOff[0] = 0;
-      for (int k = 0; k < 13; k++) nbOff[k + 1] += nbOff[k];
+      for (int k = 0; k < 26; k++) nbOff[k + 1] += nbOff[k];
     }
     // offsets within reach (gap test passed), in two passes so that the
     // touching ones (|d| <= 1 on every axis) come first; forward ones (the
@@ -1175,7 +1236,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       if (r != (uint32_t)ia) uf_union(par, ahead[ia], ahead[r]);
     }
     // (A, B): the neighbours' items in chunks of MAXRUNSB
-    const uint32_t ntot = nbOff[13];
+    const uint32_t ntot = nbOff[26];
     for (uint32_t c0 = 0; c0 < ntot; c0 += MAXRUNSB) {
       const uint32_t c1 = min(ntot, c0 + (uint32_t)MAXRUNSB);
       if (threadIdx.x == 0) cnt = 0;

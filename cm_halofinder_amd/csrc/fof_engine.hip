@@ -32,6 +32,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <type_traits>
+
 #include "fof_internal.h"
 
 namespace fof {
@@ -969,6 +971,11 @@ __device__ __forceinline__ void lunion(volatile uint32_t *lp, uint32_t a, uint32
 __device__ __forceinline__ bool runs_link(const float4 *__restrict__ pos4, uint32_t ha,
                                           uint32_t ea, uint32_t hb, uint32_t eb, const Rule &R,
                                           uint32_t &tests, uint32_t &rechecks) {
+  if (ea == ha + 1 && eb == hb + 1) {  // two single particles: both loads at once
+    const float4 a = pos4[ha], b = pos4[hb];
+    tests++;
+    return pair_linked(a, b, R, rechecks);
+  }
   for (uint32_t i = ha; i < ea; i++) {
     const float4 pi = pos4[i];
     uint32_t j = hb;
@@ -1004,7 +1011,7 @@ __device__ __forceinline__ uint16_t pack_home(int x, int y, int z) {
 // otherwise run pairs are enumerated directly.
 __device__ __forceinline__ uint32_t pack_off(int a, int b, int c) { return pack_rel(a, b, c); }
 
-template <int NT, int MAXRUNS, int MAXRUNSB, int BIGQ, int MAXOFF>
+template <int NT, int MAXRUNS, int MAXRUNSB, int BIGQ, int MAXOFF, int ROUND>
 __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ dense,
                                                     const uint32_t *__restrict__ ndense,
                                                     const float4 *__restrict__ pos4,
@@ -1024,7 +1031,10 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
   __shared__ uint16_t arel[MAXRUNS];
   __shared__ uint32_t bhead[MAXRUNSB], brel[MAXRUNSB], bgr[MAXRUNSB];
   __shared__ uint32_t lp[MAXRUNS + MAXRUNSB];
-  __shared__ uint16_t amap[kMaxRuns];  // sub-cell of A -> run slot (<= 12^3 sub-cells)
+  // sub-cell of A -> run slot (<= 12^3 sub-cells); bytes when slots fit
+  using MapT = typename std::conditional<(MAXRUNS < 256), uint8_t, uint16_t>::type;
+  constexpr uint32_t kNoRun = (MAXRUNS < 256) ? 0xFFu : 0xFFFFu;
+  __shared__ MapT amap[kMaxRuns];
   __shared__ uint32_t bigq[BIGQ];
   __shared__ uint32_t offs[MAXOFF];     // offsets within reach: touching ones first
   __shared__ uint32_t nbS[26], nbOff[27], nbTab[26];
@@ -1055,7 +1065,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       const int gx = max(abs(dx) - 1, 0), gy = max(abs(dy) - 1, 0), gz = max(abs(dz) - 1, 0);
       return (float)(gx * gx) * gs0 + (float)(gy * gy) * gs1 + (float)(gz * gz) * gs2 <= gp2;
     };
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = 0xFFFF;
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = (MapT)kNoRun;
     // neighbour table: the 13 forward cells of src_v2/FOF.c:122-153 and,
     // for a constant l on a dense grid, the 13 backward cells when sparse
     // (k_link_sparse leaves their pairs with dense cells to this kernel);
@@ -1123,7 +1133,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       arel[k] = pack_home(rx, ry, rz);
       agr[k] = uf_find(par, j);  // snapshot of the global root
       lp[k] = k;
-      amap[(rz * mm[1] + ry) * mm[0] + rx] = (uint16_t)k;
+      amap[(rz * mm[1] + ry) * mm[0] + rx] = (MapT)k;
     }
     __syncthreads();
     const int na = (int)cnt;
@@ -1191,15 +1201,16 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       if (ga == gb) lunion(lp, sa, sb);
       else test_pair(sa, sb, ha, hb);
     };
-    // (A, A) in rounds of at most BIGQ candidates, each followed by a drain,
-    // so that merges found early prune the later rounds: either every run
+    // (A, A) in rounds of ROUND candidates, each followed by a drain, so
+    // that merges found early prune the later rounds (a full queue falls
+    // back to testing in place): either every run
     // pair, or (offset-major, touching offsets first) the forward offsets
     // through the sub-cell map
     {
       const bool map = use_off && (uint32_t)na > nof / 2;
       const uint32_t total = map ? nof * (uint32_t)na : (uint32_t)na * (uint32_t)na;
-      for (uint32_t base = 0; base < total; base += BIGQ) {
-        const uint32_t pe = min(total, base + (uint32_t)BIGQ);
+      for (uint32_t base = 0; base < total; base += ROUND) {
+        const uint32_t pe = min(total, base + (uint32_t)ROUND);
         for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
           uint32_t ia, ib;
           if (map) {
@@ -1213,7 +1224,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
                       bz = (int)(ra >> 8) + dz;
             if (bx < 0 || bx >= mm[0] || by < 0 || by >= mm[1] || bz >= mm[2]) continue;
             ib = amap[(bz * mm[1] + by) * mm[0] + bx];
-            if (ib == 0xFFFF) continue;
+            if (ib == kNoRun) continue;
           } else {
             ia = p / (uint32_t)na;
             ib = p - ia * (uint32_t)na;
@@ -1271,8 +1282,8 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       // direct, or (offset-major, touching first) through the sub-cell map
       const bool map = use_off && (uint32_t)na > nof;
       const uint32_t total = map ? nof * nb : (uint32_t)na * nb;
-      for (uint32_t base = 0; base < total; base += BIGQ) {
-        const uint32_t pe = min(total, base + (uint32_t)BIGQ);
+      for (uint32_t base = 0; base < total; base += ROUND) {
+        const uint32_t pe = min(total, base + (uint32_t)ROUND);
         for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
           uint32_t ia, ib;
           if (map) {
@@ -1283,7 +1294,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
                       az = rel_z(rb) + rel_z(o);
             if (ax < 0 || ax >= mm[0] || ay < 0 || ay >= mm[1] || az < 0 || az >= mm[2]) continue;
             ia = amap[(az * mm[1] + ay) * mm[0] + ax];
-            if (ia == 0xFFFF) continue;
+            if (ia == kNoRun) continue;
           } else {
             ib = p / (uint32_t)na;
             ia = p - ib * (uint32_t)na;
@@ -2050,7 +2061,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if ((rc = te(5))) return rc;
   if (nbig) {
     if ((rc = tb(6))) return rc;
-    k_link_dense<256, kMaxRuns, 448, 512, 512><<<std::min(nbig, 8192u), 256, 0, st>>>(
+    k_link_dense<256, kMaxRuns, 448, 512, 512, 512><<<std::min(nbig, 8192u), 256, 0, st>>>(
         dbig, small + 7, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
         small + 4);
     FOF_HIP(hipGetLastError());
@@ -2058,7 +2069,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   }
   if (ndense) {
     if ((rc = tb(15))) return rc;
-    k_link_dense<64, (int)kBigCell, 128, 128, 128><<<std::min(ndense, 65536u), 64, 0, st>>>(
+    k_link_dense<64, (int)kBigCell - 1, 128, 128, 128, 1024><<<std::min(ndense, 65536u), 64, 0, st>>>(
         dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
         small + 4);
     FOF_HIP(hipGetLastError());

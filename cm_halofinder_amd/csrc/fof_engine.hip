@@ -928,8 +928,9 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
 constexpr int kMaxRuns = 1728;
 constexpr uint32_t kMaxT = 512;
 // Default dense threshold: on clumpy 1024^3 boxes (b = 0.2, about one
-// particle per cell) the call time is flat within 2% from 32 to 96.
-constexpr uint32_t kDefaultT = 48;
+// particle per cell) the call time is flat within 1% from 24 to 32 and 5%
+// higher at 48.
+constexpr uint32_t kDefaultT = 32;
 
 __device__ __forceinline__ uint32_t pack_rel(int a, int b, int c) {
   return ((uint32_t)(a & 255)) | ((uint32_t)(b & 255) << 8) | ((uint32_t)(c & 255) << 16);
@@ -2061,7 +2062,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if ((rc = te(5))) return rc;
   if (nbig) {
     if ((rc = tb(6))) return rc;
-    k_link_dense<256, kMaxRuns, 448, 512, 512, 512><<<std::min(nbig, 8192u), 256, 0, st>>>(
+    k_link_dense<256, kMaxRuns, 448, 512, 512, 4096><<<std::min(nbig, 8192u), 256, 0, st>>>(
         dbig, small + 7, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
         small + 4);
     FOF_HIP(hipGetLastError());

@@ -78,17 +78,24 @@ FWD13 = [(1, 0, 0), (-1, 1, 0), (0, 1, 0), (1, 1, 0)] + [
     (dx, dy, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
 
 
+BIG_CELL = 256  # dense cells from this many particles: the 256-thread kernel
+
+
 def candidate_pairs(recs_u8, vols, T):
     """Algorithmic work of the linking pass: the pairs the reference tests
     (src_v2/FOF.c:94-161: every pair inside a cell plus every pair with the
-    13 forward neighbours, on the reference grid of :45-65), per volume.
-    Split by the home cell's kernel: cells with at least T particles (and
-    sub-cells enabled) are k_link_dense's, the others k_link_sparse's.
-    Returns (dense_pairs, sparse_pairs, dense_cells)."""
+    13 forward neighbours, on the reference grid of :45-65), per volume,
+    charged to the kernel that tests them.  Dense cells (at least T
+    particles, sub-cells enabled) belong to k_link_dense (256-thread
+    workgroups from BIG_CELL particles, one wave below); for a constant
+    linking length a pair touching a dense cell is tested by that cell's
+    kernel (the lower cell's when both are dense), otherwise by the home
+    cell's.  Returns ({"big": .., "wave": .., "sparse": ..}, dense_cells)."""
     import torch
 
     xyz = recs_u8.view(torch.float32).view(-1, 8)[:, 2:5]
-    dp = sp = nd = 0
+    tot = {"big": 0, "wave": 0, "sparse": 0}
+    nd = 0
     for v in vols:
         d = v.domain
         a, n = int(v.offset), int(v.n)
@@ -98,6 +105,7 @@ def candidate_pairs(recs_u8, vols, T):
         s = d.linklength * (1 - 1e-6) / np.sqrt(3.0)
         m = [max(1, int(np.ceil(d.lcell[k] / s))) if s > 0 else 99 for k in range(3)]
         sub_on = all(m[k] <= 12 and dims[k] * m[k] < 2 ** 21 for k in range(3))
+        by_dense = not bool(d.varlink)  # constant l: pairs with a dense cell go to its kernel
         idx = torch.empty(n, dtype=torch.int64, device=xyz.device)
         for c0 in range(0, n, 1 << 26):
             c1 = min(n, c0 + (1 << 26))
@@ -107,7 +115,19 @@ def candidate_pairs(recs_u8, vols, T):
             idx[c0:c1] = (t[:, 2] * dims[1] + t[:, 1]) * dims[0] + t[:, 0]
         cnt = torch.bincount(idx, minlength=dims[0] * dims[1] * dims[2]).view(dims[2], dims[1], dims[0])
         del idx
-        fwd = torch.zeros_like(cnt)
+        # class of every cell: 0 sparse, 1 wave, 2 big
+        cls = torch.zeros_like(cnt, dtype=torch.int8)
+        if sub_on:
+            cls[cnt >= T] = 1
+            cls[(cnt >= T) & (cnt >= BIG_CELL)] = 2
+        nd += int((cls > 0).sum())
+        acc = torch.zeros(3, dtype=torch.float64, device=cnt.device)
+
+        def add(pairs, klass):
+            for k in range(3):
+                acc[k] += pairs[klass == k].sum().to(torch.float64)
+
+        add(cnt * (cnt - 1) // 2, cls)
 
         def sl(dd, nn):
             return (slice(0, nn - 1), slice(1, nn)) if dd == 1 else \
@@ -115,14 +135,15 @@ def candidate_pairs(recs_u8, vols, T):
 
         for dx, dy, dz in FWD13:
             (zd, zs), (yd, ys), (xd, xs) = sl(dz, dims[2]), sl(dy, dims[1]), sl(dx, dims[0])
-            fwd[zd, yd, xd] += cnt[zs, ys, xs]
-        pairs = cnt * (cnt - 1) // 2 + cnt * fwd
-        dense = (cnt >= T) if sub_on else torch.zeros_like(cnt, dtype=torch.bool)
-        dp += int(pairs[dense].sum())
-        sp += int(pairs[~dense].sum())
-        nd += int(dense.sum())
-        del cnt, fwd, pairs, dense
-    return dp, sp, nd
+            ca, cb = cnt[zd, yd, xd], cnt[zs, ys, xs]
+            ka, kb = cls[zd, yd, xd], cls[zs, ys, xs]
+            klass = torch.where(ka > 0, ka, kb) if by_dense else ka
+            add(ca * cb, klass)
+        tot["sparse"] += int(acc[0].item())
+        tot["wave"] += int(acc[1].item())
+        tot["big"] += int(acc[2].item())
+        del cnt, cls
+    return tot, nd
 
 
 def profiled_traffic(kernel, workload):
@@ -143,12 +164,12 @@ def roofline_link(stats, L, recs_u8, vols, workload=None):
     flops (8 per candidate pair of the reference sweep) over the kernel's
     device time from the library's HIP events on its stream."""
     ph = dict(zip(L.PHASES, stats.phase_ms))
-    dp, sp, nd = candidate_pairs(recs_u8, vols, stats.dense_threshold)
-    if ph["link_dense"] >= ph["link_sparse"]:
-        kern, ms, pairs = "k_link_dense", ph["link_dense"], dp
-    else:
-        kern, ms, pairs = "k_link_sparse", ph["link_sparse"], sp
-    tf = pairs * 8 / (ms * 1e-3) / 1e12
+    pairs, nd = candidate_pairs(recs_u8, vols, stats.dense_threshold)
+    cands = [("k_link_dense", "link_dense", "big"), ("k_link_dense_wave", "link_dense_wave", "wave"),
+             ("k_link_sparse", "link_sparse", "sparse")]
+    kern, phase, key = max(cands, key=lambda c: ph[c[1]])
+    ms = ph[phase]
+    tf = pairs[key] * 8 / (ms * 1e-3) / 1e12
     traffic, src = profiled_traffic(kern, workload) if workload else (None, None)
     return {
         "kernel": kern,
@@ -161,9 +182,13 @@ def roofline_link(stats, L, recs_u8, vols, workload=None):
         "traffic": traffic,
         "traffic_source": src,
         "kernel_ms": ms,
-        "algorithmic_pairs": pairs,
+        "algorithmic_pairs": pairs[key],
         "flop_per_pair": 8,
-        "pairs_by_kernel": {"k_link_dense": dp, "k_link_sparse": sp},
+        "pairs_by_kernel": {"k_link_dense": pairs["big"], "k_link_dense_wave": pairs["wave"],
+                            "k_link_sparse": pairs["sparse"]},
+        "kernel_ms_by_kernel": {"k_link_dense": ph["link_dense"],
+                                "k_link_dense_wave": ph["link_dense_wave"],
+                                "k_link_sparse": ph["link_sparse"]},
         "dense_cells_check": {"counted": nd, "engine": int(stats.ndense)},
         "pair_tests_performed": int(stats.pair_tests),
     }

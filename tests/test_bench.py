@@ -17,9 +17,13 @@ def test_candidate_pairs_match_reference_sweep():
     _, pairs = O.link_uf(P["Pos"], od)
     recs = torch.from_numpy(P.view(np.uint8).reshape(-1).copy())
     for T in (1, 12, 96, 1 << 30):
-        dp, sp, nd = bench.candidate_pairs(recs, vols, T)
-        assert dp + sp == pairs
+        by, nd = bench.candidate_pairs(recs, vols, T)
+        assert by["big"] + by["wave"] + by["sparse"] == pairs
         if T == 1 << 30:
-            assert dp == 0 and nd == 0
-    dp1, _, nd1 = bench.candidate_pairs(recs, vols, 1)
-    assert nd1 > 0 and dp1 == pairs  # every occupied cell is dense at T = 1
+            assert by["big"] == by["wave"] == 0 and nd == 0
+    by1, nd1 = bench.candidate_pairs(recs, vols, 1)
+    assert nd1 > 0 and by1["sparse"] == 0  # every occupied cell is dense at T = 1
+    # a pair touching a dense cell is charged to a dense kernel
+    by12, _ = bench.candidate_pairs(recs, vols, 12)
+    byinf, _ = bench.candidate_pairs(recs, vols, 1 << 30)
+    assert by12["sparse"] < byinf["sparse"]

@@ -271,7 +271,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pside", type=int, default=1024)
     ap.add_argument("--g", type=int, default=4, help="subvolumes per axis for the batched path")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="box", choices=["box", "precision", "volumes"],
                     help="box: the headline (1024^3, plain vs batched); precision: the same box "

@@ -244,6 +244,117 @@ def lightcone_particles(lc: Lightcone) -> np.ndarray:
     return out
 
 
+def _interp_t(x, xp, fp):
+    """np.interp for torch tensors (xp increasing, clamped at the ends)."""
+    import torch
+
+    i = torch.searchsorted(xp, x).clamp(1, len(xp) - 1)
+    x0, x1, y0, y1 = xp[i - 1], xp[i], fp[i - 1], fp[i]
+    t = ((x - x0) / (x1 - x0)).clamp(0.0, 1.0)
+    return y0 + t * (y1 - y0)
+
+
+def lightcone_particles_device(lc: Lightcone, device):
+    """The model of lightcone_particles generated on the device with torch's
+    RNG (another realization of the same model), for sizes where the host
+    version would take minutes.  Returns the 32-byte records as a uint8
+    tensor (ID, Pos, Vel as in the reference record)."""
+    import torch
+
+    f64 = torch.float64
+    g = torch.Generator(device=device)
+    g.manual_seed(lc.seed)
+    cdf = torch.tensor(lc._cdf, dtype=f64, device=device)
+    rr = torch.tensor(lc._rr, dtype=f64, device=device)
+    zr = torch.tensor(lc._r, dtype=f64, device=device)
+    zz = torch.tensor(lc._z, dtype=f64, device=device)
+
+    def sample_r(k):
+        return _interp_t(torch.rand(k, generator=g, device=device, dtype=f64), cdf, rr)
+
+    def link(r):
+        z = _interp_t(r, zr, zz)
+        return lc.b * (lc.n0 * (1 + z) ** -1.5) ** (-1.0 / 3.0)
+
+    def dirs(k):
+        v = torch.randn(k, 3, generator=g, device=device, dtype=f64)
+        return v / v.norm(dim=1, keepdim=True)
+
+    n = lc.n
+    nbg = n // 2
+    ncl = n - nbg
+    pos = torch.empty(n, 3, dtype=f64, device=device)
+    pos[:nbg] = dirs(nbg) * sample_r(nbg)[:, None]
+    # clump sizes: the quantile rule of the host generator
+    K = max(1, int(round(n * CLUMPS_PER_PARTICLE)))
+    u = (np.arange(K) + 0.5) / K
+    a1 = 1 - 1.9
+    m = (20.0 ** a1 + u * (5000.0 ** a1 - 20.0 ** a1)) ** (1 / a1)
+    sz = np.maximum(1, np.floor(m * ncl / m.sum())).astype(np.int64)
+    sz[: ncl - sz.sum()] += 1 if ncl > sz.sum() else 0
+    while sz.sum() > ncl:
+        sz[np.argmax(sz)] -= 1
+    szt = torch.tensor(sz, device=device)
+    rc = sample_r(K)
+    cen = dirs(K) * rc[:, None]
+    a = 0.12 * link(rc) * (szt.to(f64) / 100.0) ** (1.0 / 3.0)
+    who = torch.repeat_interleave(torch.arange(K, device=device), szt)
+    uu = 0.99 * torch.rand(ncl, generator=g, device=device, dtype=f64)
+    rad = a[who] / torch.sqrt(uu ** (-2.0 / 3.0) - 1)
+    pos[nbg:] = cen[who] + dirs(ncl) * rad[:, None]
+    del who, uu, rad, cen
+    rad = pos.norm(dim=1)
+    out = rad >= lc.rmax
+    pos[out] *= (lc.rmax * 0.999 / rad[out])[:, None]  # pull the few escapees inside
+    del rad, out
+    perm = torch.randperm(n, generator=g, device=device)
+    rec = torch.empty(n, 8, dtype=torch.float32, device=device)
+    rec[:, 2:5] = pos[perm].to(torch.float32)
+    del pos
+    rec[:, 5:8] = 300.0 * torch.randn(n, 3, generator=g, device=device, dtype=torch.float32)
+    rec.view(torch.int64).view(n, 4)[:, 0] = perm
+    return rec.view(torch.uint8).view(-1)
+
+
+def lightcone_shells_device(lc: Lightcone, recs):
+    """lightcone_shells on the device: one sort by radius, then each shell
+    (with its radial ghosts) is a contiguous slice, shuffled.  Returns
+    (records, volumes, owned-radius bounds, table)."""
+    import torch
+
+    n = recs.numel() // 32
+    dev = recs.device
+    pos = recs.view(torch.float32).view(n, 8)[:, 2:5]
+    r = pos.to(torch.float64).norm(dim=1)
+    rs, order = torch.sort(r)
+    del r
+    edges = lc.shells()
+    gh = lc.ghost
+    r2max, tab = lc.varlink_table()
+    vt = L.VarLinkTable([0.0, 0.0, 0.0], r2max, tab)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(lc.seed + 1)
+    rows = recs.view(n, 32)
+    parts, vols, bounds, off = [], [], [], 0
+    for k in range(lc.nshell):
+        lo, hi = float(edges[k]), float(edges[k + 1])
+        q = torch.tensor([lo - gh, hi + gh], dtype=torch.float64, device=dev)
+        i0, i1 = (int(x) for x in torch.searchsorted(rs, q).tolist())
+        idx = order[i0:i1]
+        idx = idx[torch.randperm(len(idx), generator=gen, device=dev)]
+        parts.append(rows[idx])
+        R = hi + gh
+        sep = float(lc.density(hi) ** (-1.0 / 3.0))
+        v = L.Volume()
+        v.offset, v.n = off, len(idx)
+        v.domain = L.domain([-R] * 3, [R] * 3, [-R] * 3, [R] * 3, [sep] * 3, float(lc.link(hi)),
+                            lc.nphalomin, vt)
+        off += len(idx)
+        vols.append(v)
+        bounds.append((lo, hi))
+    return torch.cat(parts).view(-1), vols, bounds, vt
+
+
 def lightcone_shells(lc: Lightcone, P: np.ndarray):
     """Radial shells with radial ghost layers as one batch.  Shell k owns
     r in [r_k, r_k+1); its domain is the cube around the origin that holds

@@ -102,3 +102,29 @@ def test_lightcone_density_and_shells():
         owned += ((rq >= lo) & (rq < hi)).sum()
         assert v.domain.linklength < v.domain.lcell[0]  # l < lcell inside every shell
     assert owned == lc.n
+
+
+def test_lightcone_device_generator_and_shells():
+    # the device generator (torch, here on the CPU) follows the host model:
+    # IDs a permutation, background radii distributed as n(r) r^2, velocities
+    # N(0, 300); the device shell builder selects what the host one does
+    import torch
+
+    lc = W.Lightcone(n=400000, nshell=8, seed=3)
+    R = W.lightcone_particles_device(lc, torch.device("cpu"))
+    P = R.numpy().view(L.PART_DTYPE)
+    assert np.array_equal(np.sort(P["ID"]), np.arange(lc.n))
+    r = np.linalg.norm(P["Pos"].astype(np.float64), axis=1)
+    assert r.max() < lc.rmax
+    bg = np.sort(r[P["ID"] < lc.n // 2])
+    u = (np.arange(len(bg)) + 0.5) / len(bg)
+    assert np.abs(bg - lc.sample_r(u)).max() < 0.02 * lc.rmax  # Kolmogorov-type bound
+    assert abs(P["Vel"].std() - 300) < 3
+    D, vols, bounds, _ = W.lightcone_shells_device(lc, R)
+    Q, vh, bh, _ = W.lightcone_shells(lc, P)
+    assert [int(v.n) for v in vols] == [int(v.n) for v in vh] and bounds == bh
+    Dq = D.numpy().view(L.PART_DTYPE)
+    for v, w in zip(vols, vh):
+        a = np.sort(Dq[int(v.offset):int(v.offset + v.n)]["ID"])
+        b = np.sort(Q[int(w.offset):int(w.offset + w.n)]["ID"])
+        assert np.array_equal(a, b)

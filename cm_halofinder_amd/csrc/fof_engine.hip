@@ -507,16 +507,31 @@ __global__ __launch_bounds__(256) void k_pc_sort(uint64_t C, const uint32_t *__r
   }
   if (threadIdx.x == 255) lc[kFineCells] = pre;  // bucket total
   __syncthreads();
-  // cell starts (coalesced) and the dense-cell lists
-  for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) {
-    const uint32_t s = lc[e], n = lc[e + 1] - s;
-    start[c0 + e] = q0 + s;
-    if (n >= T) {
-      const uint32_t cc = (uint32_t)(c0 + e);
-      if (vols[vol_of_cell(vols, nvol, cc)].m[0] > 0) {
-        if (n >= kBigCell) dbig[atomicAdd(nbig, 1u)] = cc;
-        else dense[atomicAdd(ndense, 1u)] = cc;
+  // cell starts (coalesced) and the dense-cell lists (one append per wave)
+  for (uint32_t e0 = threadIdx.x & ~63u; e0 < nc; e0 += blockDim.x) {
+    const uint32_t e = e0 + lane;
+    bool sml = false, big = false;
+    if (e < nc) {
+      const uint32_t s = lc[e], n = lc[e + 1] - s;
+      start[c0 + e] = q0 + s;
+      if (n >= T && vols[vol_of_cell(vols, nvol, (uint32_t)(c0 + e))].m[0] > 0) {
+        big = n >= kBigCell;
+        sml = !big;
       }
+    }
+    const unsigned long long bs = __ballot(sml), bb = __ballot(big);
+    const unsigned long long below = (1ull << lane) - 1;
+    if (bs) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(ndense, (uint32_t)__popcll(bs));
+      base = __shfl(base, 0, 64);
+      if (sml) dense[base + __popcll(bs & below)] = (uint32_t)(c0 + e);
+    }
+    if (bb) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(nbig, (uint32_t)__popcll(bb));
+      base = __shfl(base, 0, 64);
+      if (big) dbig[base + __popcll(bb & below)] = (uint32_t)(c0 + e);
     }
   }
   if (c0 + nc == C && threadIdx.x == 0) start[C] = q1;
@@ -639,8 +654,6 @@ __global__ __launch_bounds__(256) void k_init_uf(uint32_t N, uint32_t *__restric
 // ---------------------------------------------------------------------------
 // 4: sub-cell counting sort of dense cells; runs become cliques.
 // ---------------------------------------------------------------------------
-constexpr int kMaxBins = kMaxSub * kMaxSub * kMaxSub;  // 4096
-
 __device__ __forceinline__ int sub_bin(const VolDev &V, const float4 p) {
   double ux, uy, uz;
   const int ix = cell1((double)p.x, V.rmin_buff[0], V.lcell[0], V.nx, &ux);
@@ -649,106 +662,145 @@ __device__ __forceinline__ int sub_bin(const VolDev &V, const float4 p) {
   return (sub1(uz, iz, V.m[2]) * V.m[1] + sub1(uy, iy, V.m[1])) * V.m[0] + sub1(ux, ix, V.m[0]);
 }
 
-__global__ __launch_bounds__(256) void k_subsort(const uint32_t *__restrict__ dense,
-                                                 const uint32_t *__restrict__ ndense,
-                                                 const uint32_t *__restrict__ start,
-                                                 const VolDev *__restrict__ vols,
-                                                 const uint32_t *__restrict__ vbeg, int nvol,
-                                                 float4 *__restrict__ pos4,
-                                                 float4 *__restrict__ tmp4,
-                                                 uint32_t *__restrict__ par,
-                                                 uint32_t *__restrict__ run_end,
-                                                 unsigned long long *__restrict__ fine,
-                                                 uint32_t *__restrict__ rtoff,
-                                                 uint32_t *__restrict__ rtab,
-                                                 uint32_t *__restrict__ rtcount) {
-  __shared__ uint32_t bstart[kMaxBins];
-  __shared__ uint32_t rbase;
-  __shared__ uint32_t bcur[kMaxBins];
-  __shared__ uint32_t ws[8];
+// Sub-cell counting sort, one workgroup of NT threads per dense cell.  With
+// SMALL (cells below 256 particles, one wave) every count fits a byte:
+// counters are packed four to a word and starts are bytes, so the LDS
+// footprint stays near 4 KB; the particles' sub-cells are kept in LDS
+// between the counting and the scatter pass.
+constexpr int kMaxCellBins = 12 * 12 * 12;  // m <= 12 per axis
+
+template <int NT, bool SMALL>
+__global__ __launch_bounds__(NT) void k_subsort(const uint32_t *__restrict__ dense,
+                                                const uint32_t *__restrict__ ndense,
+                                                const uint32_t *__restrict__ start,
+                                                const VolDev *__restrict__ vols,
+                                                const uint32_t *__restrict__ vbeg, int nvol,
+                                                float4 *__restrict__ pos4,
+                                                float4 *__restrict__ tmp4,
+                                                uint32_t *__restrict__ par,
+                                                uint32_t *__restrict__ run_end,
+                                                unsigned long long *__restrict__ fine,
+                                                uint32_t *__restrict__ rtoff,
+                                                uint32_t *__restrict__ rtab) {
+  constexpr int kWaves = NT / 64;
+  // bins per thread in the scans (a multiple of 4 when counters are packed)
+  constexpr int kPer = SMALL ? ((kMaxCellBins + 4 * NT - 1) / (4 * NT)) * 4
+                             : (kMaxCellBins + NT - 1) / NT;
+  constexpr int kWords = SMALL ? kPer * NT / 4 : kMaxCellBins;
+  __shared__ uint32_t bcur[kWords];                          // cursors (packed when SMALL)
+  __shared__ uint32_t bstart32[SMALL ? 1 : kMaxCellBins];    // run starts
+  __shared__ uint8_t bstart8[SMALL ? kPer * NT : 1];
+  __shared__ uint16_t pbin[SMALL ? 256 : 1];
+  __shared__ uint32_t ws[kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  auto get_cur = [&](int b) -> uint32_t {
+    return SMALL ? (bcur[b >> 2] >> ((b & 3) * 8)) & 0xFFu : bcur[b];
+  };
+  auto get_start = [&](int b) -> uint32_t { return SMALL ? bstart8[b] : bstart32[b]; };
+  // fetch-and-increment of a bin's counter
+  auto bump = [&](int b) -> uint32_t {
+    if (SMALL) {
+      const int sh = (b & 3) * 8;
+      return (atomicAdd(&bcur[b >> 2], 1u << sh) >> sh) & 0xFFu;
+    }
+    return atomicAdd(&bcur[b], 1u);
+  };
+  // block-wide exclusive scan of one value per thread; returns (prefix, total)
+  auto block_scan = [&](uint32_t loc, uint32_t &tot) -> uint32_t {
+    uint32_t inc = loc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += t;
+    }
+    if (kWaves == 1) {
+      tot = __shfl(inc, 63, 64);
+      return inc - loc;
+    }
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+    tot = 0;
+#pragma unroll
+    for (int q = 0; q < kWaves; q++) {
+      off += (q < w) ? ws[q] : 0u;
+      tot += ws[q];
+    }
+    return off + inc - loc;
+  };
   const uint32_t nd = *ndense;
   for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
     const uint32_t c = dense[d];
-    const uint32_t s = start[c], e = start[c + 1], n = e - s;
+    const uint32_t s = start[c], e = start[c + 1], n = e - s;  // SMALL: n < 256
     const VolDev &V = vols[vol_of(vbeg, nvol, s)];
     const int nbins = V.m[0] * V.m[1] * V.m[2];
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) bcur[b] = 0;
+    for (int b = threadIdx.x; b < kWords; b += NT) bcur[b] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-      atomicAdd(&bcur[sub_bin(V, pos4[s + i])], 1u);
-    __syncthreads();
-    // exclusive scan of bcur[0..nbins) -> bstart (blocked, 16 bins/thread)
-    {
-      const int b0 = threadIdx.x * 16;
-      uint32_t loc = 0;
-      for (int q = 0; q < 16; q++)
-        if (b0 + q < nbins) loc += bcur[b0 + q];
-      uint32_t inc = loc;
-      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += t;
-      }
-      if (lane == 63) ws[w] = inc;
-      __syncthreads();
-      uint32_t off = 0;
-      for (int q = 0; q < w; q++) off += ws[q];
-      uint32_t pre = off + inc - loc;
-      for (int q = 0; q < 16; q++)
-        if (b0 + q < nbins) {
-          uint32_t cc = bcur[b0 + q];
-          bstart[b0 + q] = pre;
-          bcur[b0 + q] = pre;
-          pre += cc;
-        }
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+      const int b = sub_bin(V, pos4[s + i]);
+      if (SMALL) pbin[i] = (uint16_t)b;
+      bump(b);
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    // exclusive scan of the counts -> starts and cursors (kPer bins a thread)
+    {
+      const int b0 = threadIdx.x * kPer;
+      uint32_t loc = 0, tot;
+      for (int q = 0; q < kPer; q++)
+        if (b0 + q < nbins) loc += get_cur(b0 + q);
+      uint32_t pre = block_scan(loc, tot);
+      if (SMALL) {
+        for (int q = 0; q < kPer; q += 4) {  // whole words: no other thread shares them
+          uint32_t word = 0;
+          for (int k = 0; k < 4; k++) {
+            const int b = b0 + q + k;
+            if (b < nbins) {
+              const uint32_t cc = get_cur(b);
+              bstart8[b] = (uint8_t)pre;
+              word |= pre << (8 * k);
+              pre += cc;
+            }
+          }
+          bcur[(b0 + q) >> 2] = word;
+        }
+      } else {
+        for (int q = 0; q < kPer; q++)
+          if (b0 + q < nbins) {
+            const uint32_t cc = bcur[b0 + q];
+            bstart32[b0 + q] = pre;
+            bcur[b0 + q] = pre;
+            pre += cc;
+          }
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       const float4 p = pos4[s + i];
-      const uint32_t slot = atomicAdd(&bcur[sub_bin(V, p)], 1u);
+      const uint32_t slot = bump(SMALL ? (int)pbin[i] : sub_bin(V, p));
       tmp4[s + slot] = p;
     }
     __syncthreads();
-    // run table of the cell: [count, heads in sub-cell order], bump-allocated;
-    // rtoff at the cell's first particle points to it
+    // run table of the cell: heads in sub-cell order at rtab[s ..), their
+    // number at rtoff[s] (a cell has at most as many runs as particles)
     {
-      const int b0 = threadIdx.x * 16;
-      uint32_t loc = 0;
-      for (int q = 0; q < 16; q++)
-        if (b0 + q < nbins && bcur[b0 + q] != bstart[b0 + q]) loc++;
-      uint32_t inc = loc;
-      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += t;
-      }
-      if (lane == 63) ws[w] = inc;
-      __syncthreads();
-      uint32_t off = 0, tot = 0;
-      for (int q = 0; q < 4; q++) {
-        off += (q < w) ? ws[q] : 0u;
-        tot += ws[q];
-      }
-      if (threadIdx.x == 0) {
-        rbase = atomicAdd(rtcount, tot + 1);
-        rtab[rbase] = tot;
-        rtoff[s] = rbase;
-      }
-      __syncthreads();
-      uint32_t r = rbase + 1 + off + inc - loc;
-      for (int q = 0; q < 16; q++)
-        if (b0 + q < nbins && bcur[b0 + q] != bstart[b0 + q]) rtab[r++] = s + bstart[b0 + q];
+      const int b0 = threadIdx.x * kPer;
+      uint32_t loc = 0, tot;
+      for (int q = 0; q < kPer; q++)
+        if (b0 + q < nbins && get_cur(b0 + q) != get_start(b0 + q)) loc++;
+      const uint32_t pre = block_scan(loc, tot);
+      if (threadIdx.x == 0) rtoff[s] = tot;
+      uint32_t r = s + pre;
+      for (int q = 0; q < kPer; q++)
+        if (b0 + q < nbins && get_cur(b0 + q) != get_start(b0 + q)) rtab[r++] = s + get_start(b0 + q);
     }
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
       const float4 p = tmp4[s + i];
       const CellPos cp = cell_pos(V, p.x, p.y, p.z);
       const int b = ((cp.fz - cp.iz * V.m[2]) * V.m[1] + (cp.fy - cp.iy * V.m[1])) * V.m[0] +
                     (cp.fx - cp.ix * V.m[0]);
       pos4[s + i] = p;
-      par[s + i] = s + bstart[b];
-      run_end[s + i] = s + bcur[b];
+      par[s + i] = s + get_start(b);
+      run_end[s + i] = s + get_cur(b);
       fine[s + i] = pack_fine(cp.fx, cp.fy, cp.fz);
     }
     __syncthreads();
@@ -1038,7 +1090,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
   __shared__ MapT amap[kMaxRuns];
   __shared__ uint32_t bigq[BIGQ];
   __shared__ uint32_t offs[MAXOFF];     // offsets within reach: touching ones first
-  __shared__ uint32_t nbS[26], nbOff[27], nbTab[26];
+  __shared__ uint32_t nbS[26], nbOff[27];
   __shared__ int nbLo[26][3], nbHi[26][3];
   __shared__ uint8_t nbDense[26];
   __shared__ uint32_t cnt, qn, noff;
@@ -1090,8 +1142,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       const bool dn = (eB - sB >= T);
       nbS[k] = sB;
       nbDense[k] = dn ? 1 : 0;
-      nbTab[k] = dn ? rtoff[sB] : 0u;
-      nbOff[k + 1] = dn ? rtab[nbTab[k]] : eB - sB;
+      nbOff[k + 1] = dn ? rtoff[sB] : eB - sB;  // runs of a dense cell, else particles
       for (int a = 0; a < 3; a++) {
         int lo = -1000, hi = 1000;
         if (dd[a] == 1) lo = mm[a], hi = mm[a] - 1 + reach[a];
@@ -1123,10 +1174,9 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       }
     }
     // the home cell's runs (one per occupied sub-cell) from its run table
-    const uint32_t atab = rtoff[sA];
-    const uint32_t nruns = rtab[atab];
+    const uint32_t nruns = rtoff[sA];
     for (uint32_t t = threadIdx.x; t < nruns; t += blockDim.x) {
-      const uint32_t j = rtab[atab + 1 + t];
+      const uint32_t j = rtab[sA + t];
       const unsigned long long f = fine[j];
       const int rx = fine_x(f) - ox, ry = fine_y(f) - oy, rz = fine_z(f) - oz;
       const uint32_t k = atomicAdd(&cnt, 1u);  // runs <= min(particles, sub-cells) <= MAXRUNS
@@ -1257,7 +1307,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
         int q = 0;
         while (nbOff[q + 1] <= g) q++;
         const uint32_t item = g - nbOff[q];
-        const uint32_t j = nbDense[q] ? rtab[nbTab[q] + 1 + item] : nbS[q] + item;
+        const uint32_t j = nbDense[q] ? rtab[nbS[q] + item] : nbS[q] + item;
         int fx, fy, fz;
         if (nbDense[q]) {
           const unsigned long long f = fine[j];
@@ -1802,7 +1852,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if ((rc = ensure(W.par, N * 4 + 64))) return rc;
   if ((rc = ensure(W.runend, N * 4 + 64))) return rc;
   if ((rc = ensure(W.fine, N * 8 + 64))) return rc;
-  if ((rc = ensure(W.rtab, (N + ndense_cap + 64) * 4))) return rc;
+  if ((rc = ensure(W.rtab, N * 4 + 64))) return rc;
   if ((rc = ensure(W.gsize, N * 4 + 64))) return rc;
   if ((rc = ensure(W.label, N * 4 + 64))) return rc;
   const uint64_t q4 = (N + 63) / 64 * 64;  // u32 entries per sort array
@@ -2046,11 +2096,16 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     if ((rc = tb(4))) return rc;
     const uint32_t lists[2] = {ndense, nbig};
     uint32_t *lptr[2] = {dense, dbig};
-    for (int l = 0; l < 2; l++) {
-      if (!lists[l]) continue;
-      k_subsort<<<std::min(lists[l], 8192u), 256, 0, st>>>(
-          lptr[l], small + (l ? 7 : 1), start, dV, dVB, (int)nvol, pos4,
-          reinterpret_cast<float4 *>(sk0), par, runend, fine, rtoff, rtab, small + 6);
+    if (lists[0]) {  // cells below kBigCell particles: one wave each
+      k_subsort<64, true><<<std::min(lists[0], 65536u), 64, 0, st>>>(
+          lptr[0], small + 1, start, dV, dVB, (int)nvol, pos4, reinterpret_cast<float4 *>(sk0),
+          par, runend, fine, rtoff, rtab);
+      FOF_HIP(hipGetLastError());
+    }
+    if (lists[1]) {
+      k_subsort<256, false><<<std::min(lists[1], 8192u), 256, 0, st>>>(
+          lptr[1], small + 7, start, dV, dVB, (int)nvol, pos4, reinterpret_cast<float4 *>(sk0),
+          par, runend, fine, rtoff, rtab);
       FOF_HIP(hipGetLastError());
     }
     if ((rc = te(4))) return rc;

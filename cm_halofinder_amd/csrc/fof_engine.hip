@@ -1537,54 +1537,42 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
     const uint32_t n = hsize[lab];
     const int v = vol_of(vbeg, nvol, lab);
     const VolDev &V = vols[v];
-    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    // lanes 0-5 each accumulate one component in ring order (the host's
+    // order); the boundary test of src_v2/FOF.c:403-422 uses the staged
+    // positions of every member and is kept for owned halos only
+    double acc = 0;
+    uint32_t warn = 0;
     for (uint32_t k = 0; k < n; k += 64) {
       const uint32_t m = min(64u, n - k);
       if ((uint32_t)lane < m) {
         const uint8_t *rec = base + (uint64_t)sval[k0 + k + lane] * stride;
         const float *pp = reinterpret_cast<const float *>(rec + offpos);
         const float *vv = reinterpret_cast<const float *>(rec + offvel);
-        stage[w][0][lane] = pp[0], stage[w][1][lane] = pp[1], stage[w][2][lane] = pp[2];
+        const float px = pp[0], py = pp[1], pz = pp[2];
+        stage[w][0][lane] = px, stage[w][1][lane] = py, stage[w][2][lane] = pz;
         stage[w][3][lane] = vv[0], stage[w][4][lane] = vv[1], stage[w][5][lane] = vv[2];
+        const double q[3] = {px, py, pz};
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+          if ((q[d] - V.rmin_buff[d] < V.linklength) || (V.rmax_buff[d] - q[d] < V.linklength))
+            warn |= 1u << d;
       }
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      if (lane == 0) {
-        for (uint32_t q = 0; q < m; q++) {
-          s0 += (double)stage[w][0][q];
-          s1 += (double)stage[w][1][q];
-          s2 += (double)stage[w][2][q];
-          s3 += (double)stage[w][3][q];
-          s4 += (double)stage[w][4][q];
-          s5 += (double)stage[w][5][q];
-        }
-      }
+      if (lane < 6)
+        for (uint32_t q = 0; q < m; q++) acc += (double)stage[w][lane][q];
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
-    // broadcast lane 0's sums
-    s0 = __shfl(s0, 0, 64), s1 = __shfl(s1, 0, 64), s2 = __shfl(s2, 0, 64);
-    s3 = __shfl(s3, 0, 64), s4 = __shfl(s4, 0, 64), s5 = __shfl(s5, 0, 64);
+    const double s0 = __shfl(acc, 0, 64), s1 = __shfl(acc, 1, 64), s2 = __shfl(acc, 2, 64);
+    const double s3 = __shfl(acc, 3, 64), s4 = __shfl(acc, 4, 64), s5 = __shfl(acc, 5, 64);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) warn |= __shfl_xor(warn, o, 64);
     // src_v2/FOF.c:394-401
     const double cx = s0 / (double)n, cy = s1 / (double)n, cz = s2 / (double)n;
     const bool own = !((cx < V.rmin[0]) || (cx >= V.rmax[0]) || (cy < V.rmin[1]) ||
                        (cy >= V.rmax[1]) || (cz < V.rmin[2]) || (cz >= V.rmax[2]));
-    uint32_t warn = 0;
-    if (own) {
-      // src_v2/FOF.c:403-422
-      for (uint32_t k = lane; k < n; k += 64) {
-        const float *pp =
-            reinterpret_cast<const float *>(base + (uint64_t)sval[k0 + k] * stride + offpos);
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          const double p = pp[d];
-          if ((p - V.rmin_buff[d] < V.linklength) || (V.rmax_buff[d] - p < V.linklength))
-            warn |= 1u << d;
-        }
-      }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) warn |= __shfl_xor(warn, o, 64);
-    }
+    if (!own) warn = 0;
     if (lane == 0) {
       fof_halo_v1 H1;
       H1.head = lab - vbeg[v];

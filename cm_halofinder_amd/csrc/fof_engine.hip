@@ -1512,7 +1512,7 @@ __global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
 // ascending index order from the head (the order the host walks the ring
 // returned in next_out), so the host's Checkhalo / Output_Halos loops give
 // the same bits.  One wavefront per halo: the 64 lanes gather 64 members'
-// records at a time into LDS and lane 0 adds them in order.
+// records at a time into LDS and lanes 0-5 add one component each, in order.
 __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ dH,
                                                    const uint32_t *__restrict__ hk,
                                                    const uint32_t *__restrict__ skey,

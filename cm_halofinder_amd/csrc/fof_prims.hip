@@ -197,7 +197,9 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t *__restrict__
 
 // Stable scatter.  Wave w of the block owns elements [w*1024, w*1024+1024) of
 // the tile and walks them 64 at a time; lanes holding the same digit find one
-// another with eight ballots, so ranks follow element order.
+// another with eight ballots, so ranks follow element order.  The tile is
+// then laid out in LDS in digit order and written out digit run by digit
+// run, so that consecutive threads write consecutive addresses.
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *__restrict__ kin,
                                                        const uint32_t *__restrict__ vin,
                                                        uint32_t *__restrict__ kout,
@@ -206,13 +208,16 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *__restric
                                                        const uint32_t *__restrict__ hoff,
                                                        uint32_t nb) {
   __shared__ uint32_t wcnt[4][256];
-  __shared__ uint32_t boff[256];
+  __shared__ uint32_t boff[256];   // global start of each digit for this tile
+  __shared__ uint32_t toff[257];   // start of each digit inside the tile
+  __shared__ uint32_t lk[kTile], lv[kTile];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int q = 0; q < 4; q++) wcnt[q][threadIdx.x] = 0;
   boff[threadIdx.x] = hoff[(uint64_t)threadIdx.x * nb + blockIdx.x];
   __syncthreads();
-  const uint64_t chunk = (uint64_t)blockIdx.x * kTile + (uint64_t)w * (kTile / 4);
+  const uint64_t tbase = (uint64_t)blockIdx.x * kTile;
+  const uint64_t chunk = tbase + (uint64_t)w * (kTile / 4);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   uint32_t kk[kItems], vv[kItems], rk[kItems];
 #pragma unroll
@@ -238,24 +243,55 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *__restric
     rk[r] = valid ? old + rin : kNone;
   }
   __syncthreads();
+  // per digit: offsets of the waves inside the digit, and the digit's total
+  uint32_t dtot;
   {
     uint32_t s = 0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      uint32_t c = wcnt[q][threadIdx.x];
+      const uint32_t c = wcnt[q][threadIdx.x];
       wcnt[q][threadIdx.x] = s;
       s += c;
     }
+    dtot = s;
+  }
+  // exclusive scan of the digit totals over the block -> toff
+  {
+    uint32_t inc = dtot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += t;
+    }
+    __shared__ uint32_t wsum[4];
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) off += (q < w) ? wsum[q] : 0u;
+    toff[threadIdx.x] = off + inc - dtot;
+    if (threadIdx.x == 255) toff[256] = off + inc;
   }
   __syncthreads();
+  // tile in digit order in LDS
 #pragma unroll
   for (int r = 0; r < kItems; r++) {
     if (rk[r] != kNone) {
       const uint32_t d = (kk[r] >> shift) & 255u;
-      const uint32_t pos = boff[d] + wcnt[w][d] + rk[r];
-      kout[pos] = kk[r];
-      vout[pos] = vv[r];
+      const uint32_t j = toff[d] + wcnt[w][d] + rk[r];
+      lk[j] = kk[r];
+      lv[j] = vv[r];
     }
+  }
+  __syncthreads();
+  // write out: element j of the tile goes to its digit's run
+  const uint32_t valid = toff[256];
+  for (uint32_t j = threadIdx.x; j < valid; j += kBlock) {
+    const uint32_t k = lk[j];
+    const uint32_t d = (k >> shift) & 255u;
+    const uint32_t pos = boff[d] + (j - toff[d]);
+    kout[pos] = k;
+    vout[pos] = lv[j];
   }
 }
 

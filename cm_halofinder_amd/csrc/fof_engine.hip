@@ -359,6 +359,76 @@ __global__ __launch_bounds__(256) void k_pa_count(const uint8_t *__restrict__ ba
   for (uint32_t b = threadIdx.x; b < Bc; b += blockDim.x) hist[(uint64_t)b * ntiles + tile] = h[b];
 }
 
+// Partition scatter through LDS: a sub-tile of kSub elements is laid out in
+// bucket order in LDS, then each bucket's elements leave as one contiguous
+// run at the bucket's cursor.  cur[] holds the block's global cursors.
+constexpr int kSub = 1024;
+
+template <int NB>
+__device__ __forceinline__ void staged_part(uint32_t nsub, const uint32_t (&kk)[kSub / 256],
+                                            const float4 (&pp)[kSub / 256],
+                                            const uint32_t (&bk)[kSub / 256], uint32_t nbk,
+                                            uint32_t *cur, uint32_t *lcnt, uint32_t *loff,
+                                            uint32_t *skey, float4 *spos, uint32_t *sbk,
+                                            uint32_t *ws, uint32_t *__restrict__ okey,
+                                            float4 *__restrict__ opos) {
+  constexpr int kPer = kSub / 256;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t rank[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; k++)
+    if (threadIdx.x + k * 256 < nsub) rank[k] = atomicAdd(&lcnt[bk[k]], 1u);
+  __syncthreads();
+  // exclusive scan of lcnt over the buckets (NB / 256 per thread)
+  constexpr int kB = NB / 256;
+  uint32_t c[kB], loc = 0;
+#pragma unroll
+  for (int q = 0; q < kB; q++) {
+    const uint32_t b = threadIdx.x * kB + q;
+    c[q] = b < nbk ? lcnt[b] : 0u;
+    loc += c[q];
+  }
+  uint32_t inc = loc;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) ws[w] = inc;
+  __syncthreads();
+  uint32_t pre = inc - loc;
+  for (int q = 0; q < w; q++) pre += ws[q];
+#pragma unroll
+  for (int q = 0; q < kB; q++) {
+    const uint32_t b = threadIdx.x * kB + q;
+    if (b < nbk) loff[b] = pre;
+    pre += c[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPer; k++)
+    if (threadIdx.x + k * 256 < nsub) {
+      const uint32_t j = loff[bk[k]] + rank[k];
+      skey[j] = kk[k];
+      spos[j] = pp[k];
+      sbk[j] = bk[k];
+    }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < nsub; j += 256) {
+    const uint32_t b = sbk[j];
+    const uint32_t g = cur[b] + (j - loff[b]);
+    okey[g] = skey[j];
+    opos[g] = spos[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kB; q++) {
+    const uint32_t b = threadIdx.x * kB + q;
+    if (b < nbk) cur[b] += lcnt[b], lcnt[b] = 0;
+  }
+  __syncthreads();
+}
+
 // A, scatter: keys and {x,y,z,index} to the tile's slots of each coarse bucket.
 __global__ __launch_bounds__(256) void k_pa_scatter(const uint8_t *__restrict__ base,
                                                     uint32_t stride, uint32_t offpos, uint32_t N,
@@ -367,17 +437,35 @@ __global__ __launch_bounds__(256) void k_pa_scatter(const uint8_t *__restrict__ 
                                                     const uint32_t *__restrict__ off,
                                                     uint32_t *__restrict__ okey,
                                                     float4 *__restrict__ opos) {
-  __shared__ uint32_t cur[kMaxCoarse];
+  __shared__ uint32_t cur[kMaxCoarse], lcnt[kMaxCoarse], loff[kMaxCoarse];
+  __shared__ uint32_t skey[kSub], sbk[kSub];
+  __shared__ float4 spos[kSub];
+  __shared__ uint32_t ws[4];
   const uint32_t tile = blockIdx.x;
-  for (uint32_t b = threadIdx.x; b < Bc; b += blockDim.x) cur[b] = off[(uint64_t)b * ntiles + tile];
+  for (uint32_t b = threadIdx.x; b < kMaxCoarse; b += blockDim.x) {
+    cur[b] = b < Bc ? off[(uint64_t)b * ntiles + tile] : 0u;
+    lcnt[b] = 0;
+  }
   __syncthreads();
   const uint32_t p0 = tile * kPartTile, p1 = min(N, p0 + kPartTile);
-  for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const uint32_t c = key[p];
-    const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
-    const uint32_t slot = atomicAdd(&cur[c >> (kFineBits + 8)], 1u);
-    okey[slot] = c;
-    opos[slot] = make_float4(pp[0], pp[1], pp[2], __uint_as_float(p));
+  for (uint32_t q0 = p0; q0 < p1; q0 += kSub) {
+    const uint32_t nsub = min((uint32_t)kSub, p1 - q0);
+    uint32_t kk[kSub / 256], bk[kSub / 256];
+    float4 pp[kSub / 256];
+#pragma unroll
+    for (int k = 0; k < kSub / 256; k++) {
+      const uint32_t i = threadIdx.x + k * 256;
+      if (i < nsub) {
+        const uint32_t p = q0 + i;
+        const uint32_t c = key[p];
+        const float *fp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
+        kk[k] = c;
+        bk[k] = c >> (kFineBits + 8);
+        pp[k] = make_float4(fp[0], fp[1], fp[2], __uint_as_float(p));
+      }
+    }
+    staged_part<kMaxCoarse>(nsub, kk, pp, bk, Bc, cur, lcnt, loff, skey, spos, sbk, ws, okey,
+                            opos);
   }
 }
 
@@ -425,16 +513,29 @@ __global__ __launch_bounds__(256) void k_pb_scatter(const uint32_t *__restrict__
                                                     const float4 *__restrict__ ipos,
                                                     uint32_t *__restrict__ okey,
                                                     float4 *__restrict__ opos) {
-  __shared__ uint32_t cur[256];
+  __shared__ uint32_t cur[256], lcnt[256], loff[256];
+  __shared__ uint32_t skey[kSub], sbk[kSub];
+  __shared__ float4 spos[kSub];
+  __shared__ uint32_t ws[4];
   uint32_t c, i0, i1, nt, k;
   pb_tile(blockIdx.x, tinfo, t0, cstart, c, i0, i1, nt, k);
   cur[threadIdx.x] = off[256ull * t0[c] + (uint64_t)threadIdx.x * nt + k];
+  lcnt[threadIdx.x] = 0;
   __syncthreads();
-  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    const uint32_t key = ikey[i];
-    const uint32_t slot = atomicAdd(&cur[(key >> kFineBits) & 255u], 1u);
-    okey[slot] = key;
-    opos[slot] = ipos[i];
+  for (uint32_t q0 = i0; q0 < i1; q0 += kSub) {
+    const uint32_t nsub = min((uint32_t)kSub, i1 - q0);
+    uint32_t kk[kSub / 256], bk[kSub / 256];
+    float4 pp[kSub / 256];
+#pragma unroll
+    for (int q = 0; q < kSub / 256; q++) {
+      const uint32_t i = threadIdx.x + q * 256;
+      if (i < nsub) {
+        kk[q] = ikey[q0 + i];
+        bk[q] = (kk[q] >> kFineBits) & 255u;
+        pp[q] = ipos[q0 + i];
+      }
+    }
+    staged_part<256>(nsub, kk, pp, bk, 256, cur, lcnt, loff, skey, spos, sbk, ws, okey, opos);
   }
 }
 

@@ -574,22 +574,25 @@ __global__ __launch_bounds__(256) void k_pc_sort(uint64_t C, const uint32_t *__r
                                                  uint32_t *__restrict__ ndense,
                                                  uint32_t *__restrict__ dbig,
                                                  uint32_t *__restrict__ nbig) {
-  __shared__ uint32_t lc[kFineCells + 1];
+  // counters padded by one word per 64 so that the per-thread segments of
+  // the scan fall in different LDS banks
+  __shared__ uint32_t lc[kFineCells + kFineCells / 64 + 1];
+  auto P = [](uint32_t e) { return e + (e >> 6); };
   __shared__ uint32_t ws[4];
   const uint32_t f = blockIdx.x;
   const uint32_t q0 = fstart[f], q1 = fstart[f + 1];
   const uint64_t c0 = (uint64_t)f << kFineBits;
   const uint32_t nc = (uint32_t)min((uint64_t)kFineCells, C - c0);
-  for (uint32_t e = threadIdx.x; e < kFineCells; e += blockDim.x) lc[e] = 0;
+  for (uint32_t e = threadIdx.x; e < kFineCells + kFineCells / 64 + 1; e += blockDim.x) lc[e] = 0;
   __syncthreads();
   for (uint32_t i = q0 + threadIdx.x; i < q1; i += blockDim.x)
-    atomicAdd(&lc[ikey[i] & (kFineCells - 1)], 1u);
+    atomicAdd(&lc[P(ikey[i] & (kFineCells - 1))], 1u);
   __syncthreads();
   // exclusive scan of the 2^14 counters: 64 consecutive per thread
   constexpr uint32_t kPer = kFineCells / 256;
   const uint32_t e0 = threadIdx.x * kPer;
   uint32_t loc = 0;
-  for (uint32_t q = 0; q < kPer; q++) loc += lc[e0 + q];
+  for (uint32_t q = 0; q < kPer; q++) loc += lc[P(e0 + q)];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t inc = loc;
 #pragma unroll
@@ -602,18 +605,18 @@ __global__ __launch_bounds__(256) void k_pc_sort(uint64_t C, const uint32_t *__r
   uint32_t pre = inc - loc;
   for (int q = 0; q < w; q++) pre += ws[q];
   for (uint32_t q = 0; q < kPer; q++) {
-    const uint32_t v = lc[e0 + q];
-    lc[e0 + q] = pre;
+    const uint32_t v = lc[P(e0 + q)];
+    lc[P(e0 + q)] = pre;
     pre += v;
   }
-  if (threadIdx.x == 255) lc[kFineCells] = pre;  // bucket total
+  if (threadIdx.x == 255) lc[P(kFineCells)] = pre;  // bucket total
   __syncthreads();
   // cell starts (coalesced) and the dense-cell lists (one append per wave)
   for (uint32_t e0 = threadIdx.x & ~63u; e0 < nc; e0 += blockDim.x) {
     const uint32_t e = e0 + lane;
     bool sml = false, big = false;
     if (e < nc) {
-      const uint32_t s = lc[e], n = lc[e + 1] - s;
+      const uint32_t s = lc[P(e)], n = lc[P(e + 1)] - s;
       start[c0 + e] = q0 + s;
       if (n >= T && vols[vol_of_cell(vols, nvol, (uint32_t)(c0 + e))].m[0] > 0) {
         big = n >= kBigCell;
@@ -639,7 +642,7 @@ __global__ __launch_bounds__(256) void k_pc_sort(uint64_t C, const uint32_t *__r
   __syncthreads();
   // scatter with LDS cursors
   for (uint32_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
-    const uint32_t slot = q0 + atomicAdd(&lc[ikey[i] & (kFineCells - 1)], 1u);
+    const uint32_t slot = q0 + atomicAdd(&lc[P(ikey[i] & (kFineCells - 1))], 1u);
     pos4[slot] = ipos[i];
   }
 }

@@ -248,10 +248,30 @@ __device__ __forceinline__ uint32_t uf_find(uint32_t *par, uint32_t x) {
   return cur;
 }
 
+// The roots of a and b, the two walks advancing together so that their
+// loads overlap (path halving on both).
+__device__ __forceinline__ void uf_find2(uint32_t *par, uint32_t &a, uint32_t &b) {
+  uint32_t ca = ld_par(par + a), cb = ld_par(par + b);
+  uint32_t qa = a, qb = b;
+  bool fa = (ca == a), fb = (cb == b);
+  while (!fa || !fb) {
+    const uint32_t na = fa ? ca : ld_par(par + ca);
+    const uint32_t nb = fb ? cb : ld_par(par + cb);
+    if (!fa) {
+      if (ca > na) st_par(par + qa, na), qa = ca, ca = na;
+      else fa = true;
+    }
+    if (!fb) {
+      if (cb > nb) st_par(par + qb, nb), qb = cb, cb = nb;
+      else fb = true;
+    }
+  }
+  a = ca, b = cb;
+}
+
 __device__ __forceinline__ void uf_union(uint32_t *par, uint32_t a, uint32_t b) {
   for (;;) {
-    a = uf_find(par, a);
-    b = uf_find(par, b);
+    uf_find2(par, a, b);
     if (a == b) return;
     if (a > b) {
       uint32_t t = a;
@@ -1198,6 +1218,10 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
   __shared__ int nbLo[26][3], nbHi[26][3];
   __shared__ uint8_t nbDense[26];
   __shared__ uint32_t cnt, qn, noff;
+  // one-wave cells: run sizes in LDS (home runs fit a byte: < 256 particles)
+  constexpr bool SIZES = (NT == 64);
+  __shared__ uint8_t asz[SIZES ? MAXRUNS : 1];
+  __shared__ uint16_t bsz[SIZES ? MAXRUNSB : 1];
   uint32_t tests = 0, rechecks = 0;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nd = *ndense;
@@ -1286,6 +1310,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       const uint32_t k = atomicAdd(&cnt, 1u);  // runs <= min(particles, sub-cells) <= MAXRUNS
       ahead[k] = j;
       arel[k] = pack_home(rx, ry, rz);
+      if (SIZES) asz[k] = (uint8_t)(run_end[j] - j);
       agr[k] = uf_find(par, j);  // snapshot of the global root
       lp[k] = k;
       amap[(rz * mm[1] + ry) * mm[0] + rx] = (MapT)k;
@@ -1296,7 +1321,14 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
     // a run pair (LDS slots sa, sb with heads ha, hb) not yet joined: small
     // pairs are tested here, big ones go to the wave queue
     auto test_pair = [&](uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb) {
-      const uint32_t ea = run_end[ha], eb = run_end[hb];
+      uint32_t ea, eb;
+      if (SIZES) {
+        const uint32_t zb = sb < (uint32_t)na ? (uint32_t)asz[sb] : (uint32_t)bsz[sb - na];
+        ea = ha + asz[sa];
+        eb = (zb == 65535u) ? run_end[hb] : hb + zb;
+      } else {
+        ea = run_end[ha], eb = run_end[hb];
+      }
       if ((ea - ha) * (eb - hb) > kBigPair) {
         const uint32_t q = atomicAdd(&qn, 1u);
         if (q < (uint32_t)BIGQ) {
@@ -1352,6 +1384,18 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
     // else test
     auto consider = [&](uint32_t sa, uint32_t sb, uint32_t ga, uint32_t gb, uint32_t ha,
                         uint32_t hb) {
+      if (SIZES) {  // two single particles: one test is cheaper than the LDS walks
+        const uint32_t zb = sb < (uint32_t)na ? (uint32_t)asz[sb] : (uint32_t)bsz[sb - na];
+        if (asz[sa] == 1 && zb == 1) {
+          if (ga != gb) {
+            const float4 a = pos4[ha], b = pos4[hb];
+            tests++;
+            if (!pair_linked(a, b, R, rechecks)) return;
+          }
+          lunion(lp, sa, sb);
+          return;
+        }
+      }
       if (lfind(lp, sa) == lfind(lp, sb)) return;
       if (ga == gb) lunion(lp, sa, sb);
       else test_pair(sa, sb, ha, hb);
@@ -1396,11 +1440,6 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
         drain();
       }
     }
-    // publish the home cell's merges before the neighbour pass
-    for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
-      const uint32_t r = lfind(lp, ia);
-      if (r != (uint32_t)ia) uf_union(par, ahead[ia], ahead[r]);
-    }
     // (A, B): the neighbours' items in chunks of MAXRUNSB
     const uint32_t ntot = nbOff[26];
     for (uint32_t c0 = 0; c0 < ntot; c0 += MAXRUNSB) {
@@ -1428,6 +1467,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
         const uint32_t k = atomicAdd(&cnt, 1u);  // < MAXRUNSB: one per item at most
         bhead[k] = j;
         brel[k] = pack_rel(rx, ry, rz);
+        if (SIZES) bsz[k] = nbDense[q] ? (uint16_t)min(run_end[j] - j, 65535u) : (uint16_t)1;
         bgr[k] = uf_find(par, j);
         lp[na + k] = na + k;
       }
@@ -1467,9 +1507,9 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
         const uint32_t r = lfind(lp, na + ib);
         if (r != na + ib) uf_union(par, bhead[ib], ahead[r]);
       }
-      __syncthreads();
+      if (c1 < ntot) __syncthreads();  // the slots are reused by the next chunk
     }
-    // commit the home cell's runs (merges made through neighbours)
+    // commit the home cell's runs (all their merges)
     for (int ia = threadIdx.x; ia < na; ia += blockDim.x) {
       const uint32_t r = lfind(lp, ia);
       if (r != (uint32_t)ia) uf_union(par, ahead[ia], ahead[r]);

@@ -584,7 +584,9 @@ __device__ __forceinline__ int vol_of_cell(const VolDev *__restrict__ vols, int 
 }
 
 // C: one workgroup per fine bucket, all counting in LDS.
-__global__ __launch_bounds__(256) void k_pc_sort(uint64_t C, const uint32_t *__restrict__ fstart,
+constexpr int kPcThreads = 512;
+
+__global__ __launch_bounds__(kPcThreads) void k_pc_sort(uint64_t C, const uint32_t *__restrict__ fstart,
                                                  const uint32_t *__restrict__ ikey,
                                                  const float4 *__restrict__ ipos,
                                                  uint32_t *__restrict__ start,
@@ -598,7 +600,7 @@ __global__ __launch_bounds__(256) void k_pc_sort(uint64_t C, const uint32_t *__r
   // the scan fall in different LDS banks
   __shared__ uint32_t lc[kFineCells + kFineCells / 64 + 1];
   auto P = [](uint32_t e) { return e + (e >> 6); };
-  __shared__ uint32_t ws[4];
+  __shared__ uint32_t ws[kPcThreads / 64];
   const uint32_t f = blockIdx.x;
   const uint32_t q0 = fstart[f], q1 = fstart[f + 1];
   const uint64_t c0 = (uint64_t)f << kFineBits;
@@ -609,7 +611,7 @@ __global__ __launch_bounds__(256) void k_pc_sort(uint64_t C, const uint32_t *__r
     atomicAdd(&lc[P(ikey[i] & (kFineCells - 1))], 1u);
   __syncthreads();
   // exclusive scan of the 2^14 counters: 64 consecutive per thread
-  constexpr uint32_t kPer = kFineCells / 256;
+  constexpr uint32_t kPer = kFineCells / kPcThreads;
   const uint32_t e0 = threadIdx.x * kPer;
   uint32_t loc = 0;
   for (uint32_t q = 0; q < kPer; q++) loc += lc[P(e0 + q)];
@@ -629,7 +631,7 @@ __global__ __launch_bounds__(256) void k_pc_sort(uint64_t C, const uint32_t *__r
     lc[P(e0 + q)] = pre;
     pre += v;
   }
-  if (threadIdx.x == 255) lc[P(kFineCells)] = pre;  // bucket total
+  if (threadIdx.x == kPcThreads - 1) lc[P(kFineCells)] = pre;  // bucket total
   __syncthreads();
   // cell starts (coalesced) and the dense-cell lists (one append per wave)
   for (uint32_t e0 = threadIdx.x & ~63u; e0 < nc; e0 += blockDim.x) {
@@ -2163,7 +2165,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     }
     k_pb_starts<<<grid_for(F + 1), 256, 0, st>>>(F, t0, cst, offB, NN, fst);
     FOF_HIP(hipGetLastError());
-    k_pc_sort<<<F, 256, 0, st>>>((uint64_t)C, fst, k2, p2, start, pos4, dV, (int)nvol, T, dense,
+    k_pc_sort<<<F, kPcThreads, 0, st>>>((uint64_t)C, fst, k2, p2, start, pos4, dV, (int)nvol, T, dense,
                                  small + 1, dbig, small + 7);
     FOF_HIP(hipGetLastError());
     if ((rc = te(2))) return rc;

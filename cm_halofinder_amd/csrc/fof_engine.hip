@@ -1779,6 +1779,76 @@ struct HostVol {
   double ncell_d;
 };
 
+// Smallest l^2 of the cells holding particles, per volume with a variable
+// l (as the bits of a non-negative double: their order is the doubles').
+__global__ __launch_bounds__(256) void k_varlink_min(const uint8_t *__restrict__ base,
+                                                     uint32_t stride, uint32_t offpos, uint32_t N,
+                                                     const VolDev *__restrict__ vols,
+                                                     const uint32_t *__restrict__ vbeg, int nvol,
+                                                     unsigned long long *__restrict__ vmin) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t b0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b0 < N;
+       b0 += gridDim.x * blockDim.x) {
+    const uint32_t p = b0 + lane;
+    int v = -1;
+    unsigned long long bits = ~0ull;
+    if (p < N) {
+      v = vol_of(vbeg, nvol, p);
+      const VolDev &V = vols[v];
+      const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
+      const double x = pp[0], y = pp[1], z = pp[2];
+      const bool ok = V.vntab > 0 && x >= V.rmin_buff[0] && x <= V.rmax_buff[0] &&
+                      y >= V.rmin_buff[1] && y <= V.rmax_buff[1] && z >= V.rmin_buff[2] &&
+                      z <= V.rmax_buff[2];
+      if (ok) {
+        double u;
+        const int ix = cell1(x, V.rmin_buff[0], V.lcell[0], V.nx, &u);
+        const int iy = cell1(y, V.rmin_buff[1], V.lcell[1], V.ny, &u);
+        const int iz = cell1(z, V.rmin_buff[2], V.lcell[2], V.nz, &u);
+        const double l2 = cell_linksq(V, ix, iy, iz);
+        memcpy(&bits, &l2, 8);
+      }
+    }
+    const int v0 = __shfl(v, 0, 64);
+    if (__ballot(v != v0) == 0ull) {  // one volume: reduce over the wave first
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_xor(bits, o, 64);
+        bits = t < bits ? t : bits;
+      }
+      if (lane == 0 && bits != ~0ull) atomicMin(vmin + v0, bits);
+    } else if (bits != ~0ull) {
+      atomicMin(vmin + v, bits);
+    }
+  }
+}
+
+// Sub-cells: side <= l_min (1 - 1e-6) / sqrt(3) so that any two particles
+// of one sub-cell link (see file header).  Disabled (m = 0) when more than
+// 12 per axis would be needed, or l_min = 0.
+static void set_subcells(VolDev &V, double lmin2) {
+  V.m[0] = V.m[1] = V.m[2] = 0;
+  if (!(lmin2 > 0)) return;
+  const double s = sqrt(lmin2) * (1.0 - 1e-6) / sqrt(3.0);
+  int m[3];
+  bool ok = true;
+  for (int k = 0; k < 3; k++) {
+    double q = ceil(V.lcell[k] / s);
+    if (q < 1) q = 1;
+    if (q > 12) ok = false;
+    m[k] = (int)q;
+  }
+  const int nn[3] = {V.nx, V.ny, V.nz};
+  for (int k = 0; k < 3; k++)
+    if ((double)nn[k] * m[k] >= 2097152.0) ok = false;  // 21-bit fine coordinates
+  if (ok)
+    for (int k = 0; k < 3; k++) {
+      V.m[k] = m[k];
+      const double side = V.lcell[k] / m[k];
+      V.s2[k] = side * side;
+    }
+}
+
 int prepare_volume(const fof_volume_v1 &in, double band, HostVol &hv, std::vector<const fof_varlink_v1 *> &vl) {
   const fof_domain_v1 &D = in.domain;
   VolDev &V = hv.d;
@@ -1815,32 +1885,9 @@ int prepare_volume(const fof_volume_v1 &in, double band, HostVol &hv, std::vecto
     V.vntab = L->ntab;
     V.vr2max = L->r2max;
     for (int k = 0; k < 3; k++) V.vorigin[k] = L->origin[k];
-    lmin2 = -1;  // filled by the caller once the table is on the host
+    lmin2 = -1;  // set by the caller from the occupied cells
   }
-  // Sub-cells: side <= l_min (1 - 1e-6) / sqrt(3) so that any two particles
-  // of one sub-cell link (see file header).  Disabled when more than
-  // kMaxSub per axis would be needed, or l = 0.
-  V.m[0] = V.m[1] = V.m[2] = 0;
-  if (lmin2 > 0) {
-    const double s = sqrt(lmin2) * (1.0 - 1e-6) / sqrt(3.0);
-    int m[3];
-    bool ok = true;
-    for (int k = 0; k < 3; k++) {
-      double q = ceil(D.lcell[k] / s);
-      if (q < 1) q = 1;
-      if (q > 12) ok = false;
-      m[k] = (int)q;
-    }
-    const int nn[3] = {V.nx, V.ny, V.nz};
-    for (int k = 0; k < 3; k++)
-      if ((double)nn[k] * m[k] >= 2097152.0) ok = false;  // 21-bit fine coordinates
-    if (ok)
-      for (int k = 0; k < 3; k++) {
-        V.m[k] = m[k];
-        const double side = D.lcell[k] / m[k];
-        V.s2[k] = side * side;
-      }
-  }
+  set_subcells(V, lmin2);
   return 0;
 }
 
@@ -1930,32 +1977,8 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
         FOF_HIP(hipMemcpy(&vt_host[o], L->l2tab, L->ntab * sizeof(double), hipMemcpyDeviceToHost));
       else
         memcpy(&vt_host[o], L->l2tab, L->ntab * sizeof(double));
-      double lmin2 = vt_host[o];
-      for (uint32_t k = 0; k < L->ntab; k++) {
+      for (uint32_t k = 0; k < L->ntab; k++)
         if (!(vt_host[o + k] >= 0) || !std::isfinite(vt_host[o + k])) return FOF_E_ARG;
-        lmin2 = std::min(lmin2, vt_host[o + k]);
-      }
-      VolDev &V = hv[v].d;
-      if (lmin2 > 0) {
-        const double s = sqrt(lmin2) * (1.0 - 1e-6) / sqrt(3.0);
-        bool ok = true;
-        int m[3];
-        for (int k = 0; k < 3; k++) {
-          double q = ceil(V.lcell[k] / s);
-          if (q < 1) q = 1;
-          if (q > 12) ok = false;
-          m[k] = (int)q;
-        }
-        const int nn[3] = {V.nx, V.ny, V.nz};
-        for (int k = 0; k < 3; k++)
-          if ((double)nn[k] * m[k] >= 2097152.0) ok = false;  // 21-bit fine coordinates
-        if (ok)
-          for (int k = 0; k < 3; k++) {
-            V.m[k] = m[k];
-            const double side = V.lcell[k] / m[k];
-            V.s2[k] = side * side;
-          }
-      }
       o += L->ntab;
     }
   }
@@ -2027,7 +2050,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if (!want_ring) d_next = nullptr;
 
   // volume table: VolDev[nvol] then the varlink tables
-  {
+  auto upload_vols = [&]() -> int {
     std::vector<uint8_t> blob(nvol * sizeof(VolDev) + vt_entries * sizeof(double));
     double *dtab = reinterpret_cast<double *>(static_cast<uint8_t *>(W.vols.ptr) + nvol * sizeof(VolDev));
     size_t o = 0;
@@ -2043,6 +2066,29 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     FOF_HIP(hipMemcpyAsync(W.vols.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
     FOF_HIP(hipMemcpyAsync(W.vbeg.ptr, vbeg.data(), (nvol + 1) * 4, hipMemcpyHostToDevice, st));
     FOF_HIP(hipStreamSynchronize(st));  // blob / vbeg are stack-owned
+    return 0;
+  };
+  if ((rc = upload_vols())) return rc;
+  if (vt_entries && N) {
+    // A variable l: the sub-cell size follows the smallest l of any occupied
+    // cell (only occupied cells are home cells of a pair), not of the table.
+    unsigned long long *dmin = static_cast<unsigned long long *>(W.hoff.ptr);
+    FOF_HIP(hipMemsetAsync(dmin, 0xFF, nvol * 8, st));
+    k_varlink_min<<<grid_for(N), 256, 0, st>>>(dbase, P->stride, P->off_pos, (uint32_t)N,
+                                               static_cast<const VolDev *>(W.vols.ptr),
+                                               static_cast<const uint32_t *>(W.vbeg.ptr),
+                                               (int)nvol, dmin);
+    FOF_HIP(hipGetLastError());
+    std::vector<unsigned long long> hmin(nvol);
+    FOF_HIP(hipMemcpyAsync(hmin.data(), dmin, nvol * 8, hipMemcpyDeviceToHost, st));
+    FOF_HIP(hipStreamSynchronize(st));
+    for (uint32_t v = 0; v < nvol; v++) {
+      if (!vols[v].domain.varlink || hmin[v] == ~0ull) continue;
+      double l2;
+      memcpy(&l2, &hmin[v], 8);
+      set_subcells(hv[v].d, l2);
+    }
+    if ((rc = upload_vols())) return rc;
   }
 
   const VolDev *dV = static_cast<const VolDev *>(W.vols.ptr);

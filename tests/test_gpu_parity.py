@@ -352,3 +352,21 @@ def test_lightcone_shells(ctx, flags):
         a, b = int(vol.offset), int(vol.offset + vol.n)
         olab, _ = O.link_uf(P["Pos"][a:b], odomain(vol), ovl)
         assert np.array_equal(lab[a:b], olab), v
+
+
+def test_varlink_subcells_follow_occupied_cells(ctx):
+    # The outermost lightcone shell: its table reaches r = 0, where l is
+    # smallest (about 1.6x below l at the shell), which would call for more
+    # than 12 sub-cells per axis; only occupied cells are home cells, so the
+    # sub-cell size follows their smallest l and dense cells keep sub-cells.
+    lc = W.Lightcone(n=150000, nshell=8, seed=3)
+    P0 = W.lightcone_particles(lc)
+    P, vols, bounds, vt = W.lightcone_shells(lc, P0)
+    v = vols[-1]
+    a, b = int(v.offset), int(v.offset + v.n)
+    Q = P[a:b].copy()
+    lab, nxt, halos, st = gpu_link(ctx, Q, v.domain, T=12)
+    assert st.ndense > 0
+    r2max, tab = lc.varlink_table()
+    olab, _ = O.link_uf(Q["Pos"], odomain(v), O.varlink([0.0, 0.0, 0.0], r2max, tab))
+    assert np.array_equal(lab, olab)

@@ -133,8 +133,8 @@ typedef struct fof_halo_v1 {
 /* Counters and timings.  With FOF_OPT_TIMING, phase_ms[k] is the device time
  * (HIP events on the library's stream) of span k of the last call, 0 when the
  * span did not run:
- *   0 cell keys (k_keys)            1 cell starts (scan) or key sort (sparse)
- *   2 counting-sort scatter         3 union-find init
+ *   0 cell keys                     1 bucket offsets (scan) or key sort (sparse)
+ *   2 partitions + bucket sort      3 union-find init
  *   4 sub-cell sort of dense cells  5 pair tests, sparse cells
  *   6 pair tests, dense cells of    7 union-find flatten + group minima
  *     256 particles or more

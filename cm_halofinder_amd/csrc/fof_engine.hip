@@ -1185,10 +1185,17 @@ __device__ __forceinline__ uint16_t pack_home(int x, int y, int z) {
   return (uint16_t)(x | (y << 4) | (z << 8));
 }
 
-// Sub-cell offsets within reach are listed (packed 8 bits per axis) when
-// there are at most MAXOFF of them (5^3 - 1 = 124 at the usual reach of 2);
-// otherwise run pairs are enumerated directly.
+// Candidate enumeration in the dense kernels.  The 256-thread kernel lists
+// the sub-cell offsets within reach (packed 8 bits per axis, at most MAXOFF:
+// 5^3 - 1 = 124 at the usual reach of 2) and walks them offset-major,
+// touching offsets first, when the cell has enough runs; the one-wave kernel
+// (MAXOFF = 0) walks a bitmap of A's occupied sub-cells row by row when the
+// cell has more than kMapRuns runs.  Otherwise every run pair is tested.
 __device__ __forceinline__ uint32_t pack_off(int a, int b, int c) { return pack_rel(a, b, c); }
+constexpr int kMapRuns = 24;
+
+// 0, 1, -1, 2, -2, ...: window rows visited from the centre out.
+__device__ __forceinline__ int zig(int k) { return (k & 1) ? (k + 1) >> 1 : -(k >> 1); }
 
 template <int NT, int MAXRUNS, int MAXRUNSB, int BIGQ, int MAXOFF, int ROUND>
 __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ dense,
@@ -1215,7 +1222,10 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
   constexpr uint32_t kNoRun = (MAXRUNS < 256) ? 0xFFu : 0xFFFFu;
   __shared__ MapT amap[kMaxRuns];
   __shared__ uint32_t bigq[BIGQ];
-  __shared__ uint32_t offs[MAXOFF];     // offsets within reach: touching ones first
+  constexpr bool OFFS = MAXOFF > 0;
+  constexpr int kOccW = OFFS ? 1 : kMaxRuns / 64 + 1;
+  __shared__ uint32_t offs[OFFS ? MAXOFF : 1];  // offsets within reach: touching ones first
+  __shared__ unsigned long long occ[kOccW];       // occupied sub-cells of A
   __shared__ uint32_t nbS[26], nbOff[27];
   __shared__ int nbLo[26][3], nbHi[26][3];
   __shared__ uint8_t nbDense[26];
@@ -1281,20 +1291,19 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       }
     }
     if (threadIdx.x == 0) cnt = 0, qn = 0, noff = 0;
+    if (!OFFS && threadIdx.x < kOccW) occ[threadIdx.x] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
       nbOff[0] = 0;
       for (int k = 0; k < 26; k++) nbOff[k + 1] += nbOff[k];
     }
     // offsets within reach (gap test passed), in two passes so that the
-    // touching ones (|d| <= 1 on every axis) come first; forward ones (the
-    // lexicographically positive half) are counted in nfwd
+    // touching ones (|d| <= 1 on every axis) come first
     const int wx = 2 * reach[0] + 1, wy = 2 * reach[1] + 1, wz = 2 * reach[2] + 1;
-    const int wall = wx * wy * wz;
-    const bool use_off = wall <= MAXOFF + 1;  // the zero offset is not listed
+    const bool use_off = OFFS && wx * wy * wz <= MAXOFF + 1;  // the zero offset is not listed
     if (use_off) {
       for (int pass = 0; pass < 2; pass++) {
-        for (int t = threadIdx.x; t < wall; t += blockDim.x) {
+        for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
           const int dx = t % wx - reach[0], dy = (t / wx) % wy - reach[1], dz = t / (wx * wy) - reach[2];
           const bool touch = abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1;
           if ((pass == 0) != touch || (dx == 0 && dy == 0 && dz == 0) || !gap_ok(dx, dy, dz)) continue;
@@ -1315,11 +1324,21 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       if (SIZES) asz[k] = (uint8_t)(run_end[j] - j);
       agr[k] = uf_find(par, j);  // snapshot of the global root
       lp[k] = k;
-      amap[(rz * mm[1] + ry) * mm[0] + rx] = (MapT)k;
+      const int bin = (rz * mm[1] + ry) * mm[0] + rx;
+      amap[bin] = (MapT)k;
+      if (!OFFS) atomicOr(&occ[bin >> 6], 1ull << (bin & 63));
     }
     __syncthreads();
     const int na = (int)cnt;
+    // bits of the occupied sub-cells [bin0, bin0 + len) of A (len <= 25)
+    auto rowbits = [&](int bin0, int len) -> uint32_t {
+      const int w = bin0 >> 6, sh = bin0 & 63;
+      unsigned long long v = occ[w] >> sh;
+      if (sh + len > 64) v |= occ[w + 1] << (64 - sh);
+      return (uint32_t)(v & ((1ull << len) - 1));
+    };
     const uint32_t nof = use_off ? noff : 0u;
+    const bool bmap = !OFFS && na > kMapRuns;
     // a run pair (LDS slots sa, sb with heads ha, hb) not yet joined: small
     // pairs are tested here, big ones go to the wave queue
     auto test_pair = [&](uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb) {
@@ -1402,42 +1421,71 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       if (ga == gb) lunion(lp, sa, sb);
       else test_pair(sa, sb, ha, hb);
     };
-    // (A, A) in rounds of ROUND candidates, each followed by a drain, so
-    // that merges found early prune the later rounds (a full queue falls
-    // back to testing in place): either every run
-    // pair, or (offset-major, touching offsets first) the forward offsets
-    // through the sub-cell map
-    {
-      const bool map = use_off && (uint32_t)na > nof / 2;
-      const uint32_t total = map ? nof * (uint32_t)na : (uint32_t)na * (uint32_t)na;
+    // (A, A), in rounds each followed by a drain, so that merges found early
+    // prune the later rounds (a full queue falls back to testing in place)
+    if (use_off && (uint32_t)na > nof / 2) {
+      // items (offset, run), offset-major: the forward offsets through the map
+      const uint32_t total = nof * (uint32_t)na;
       for (uint32_t base = 0; base < total; base += ROUND) {
         const uint32_t pe = min(total, base + (uint32_t)ROUND);
         for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
-          uint32_t ia, ib;
-          if (map) {
-            const uint32_t oi = p / (uint32_t)na;
-            ia = p - oi * (uint32_t)na;
-            const uint32_t o = offs[oi];
-            const int dx = rel_x(o), dy = rel_y(o), dz = rel_z(o);
-            if (!(dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0))))) continue;  // forward
-            const uint32_t ra = arel[ia];
-            const int bx = (int)(ra & 15) + dx, by = (int)((ra >> 4) & 15) + dy,
-                      bz = (int)(ra >> 8) + dz;
-            if (bx < 0 || bx >= mm[0] || by < 0 || by >= mm[1] || bz >= mm[2]) continue;
-            ib = amap[(bz * mm[1] + by) * mm[0] + bx];
-            if (ib == kNoRun) continue;
-          } else {
-            ia = p / (uint32_t)na;
-            ib = p - ia * (uint32_t)na;
-            if (ib <= ia) continue;
-            const uint32_t ra = arel[ia], rb = arel[ib];
-            if (!gap_ok((int)(rb & 15) - (int)(ra & 15),
-                        (int)((rb >> 4) & 15) - (int)((ra >> 4) & 15),
-                        (int)(rb >> 8) - (int)(ra >> 8)))
-              continue;
-          }
+          const uint32_t oi = p / (uint32_t)na, ia = p - oi * (uint32_t)na;
+          const uint32_t o = offs[oi];
+          const int dx = rel_x(o), dy = rel_y(o), dz = rel_z(o);
+          if (!(dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0))))) continue;  // forward
+          const uint32_t ra = arel[ia];
+          const int bx = (int)(ra & 15) + dx, by = (int)((ra >> 4) & 15) + dy, bz = (int)(ra >> 8) + dz;
+          if (bx < 0 || bx >= mm[0] || by < 0 || by >= mm[1] || bz >= mm[2]) continue;
+          const uint32_t ib = amap[(bz * mm[1] + by) * mm[0] + bx];
+          if (ib == kNoRun) continue;
           const uint32_t s0 = min(ia, ib), s1 = max(ia, ib);
           consider(s0, s1, agr[s0], agr[s1], ahead[s0], ahead[s1]);
+        }
+        drain();
+      }
+    } else if (bmap) {
+      // items (row of the forward window, run), row-major, rows centre-out
+      const uint32_t nrow = (uint32_t)((reach[2] + 1) * wy);
+      const uint32_t total = nrow * (uint32_t)na;
+      for (uint32_t base = 0; base < total; base += ROUND) {
+        const uint32_t pe = min(total, base + (uint32_t)ROUND);
+        for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
+          const uint32_t ri = p / (uint32_t)na, ia = p - ri * (uint32_t)na;
+          const int dz = (int)(ri / (uint32_t)wy), dy = zig((int)(ri % (uint32_t)wy));
+          if (dz == 0 && dy < 0) continue;  // not forward
+          const uint32_t ra = arel[ia];
+          const int ax = ra & 15, ay = (ra >> 4) & 15, az = ra >> 8;
+          const int by = ay + dy, bz = az + dz;
+          if (bz >= mm[2] || by < 0 || by >= mm[1] || !gap_ok(0, dy, dz)) continue;
+          const int x0 = (dz == 0 && dy == 0) ? ax + 1 : max(ax - reach[0], 0);
+          const int x1 = min(ax + reach[0], mm[0] - 1);
+          if (x0 > x1) continue;
+          const int rowb = (bz * mm[1] + by) * mm[0];
+          uint32_t bits = rowbits(rowb + x0, x1 - x0 + 1);
+          while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            if (!gap_ok(x0 + b - ax, dy, dz)) continue;
+            const uint32_t ib = amap[rowb + x0 + b];
+            const uint32_t s0 = min(ia, ib), s1 = max(ia, ib);
+            consider(s0, s1, agr[s0], agr[s1], ahead[s0], ahead[s1]);
+          }
+        }
+        drain();
+      }
+    } else {
+      const uint32_t total = (uint32_t)na * (uint32_t)na;
+      for (uint32_t base = 0; base < total; base += ROUND) {
+        const uint32_t pe = min(total, base + (uint32_t)ROUND);
+        for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
+          const uint32_t ia = p / (uint32_t)na, ib = p - ia * (uint32_t)na;
+          if (ib <= ia) continue;
+          const uint32_t ra = arel[ia], rb = arel[ib];
+          if (!gap_ok((int)(rb & 15) - (int)(ra & 15),
+                      (int)((rb >> 4) & 15) - (int)((ra >> 4) & 15),
+                      (int)(rb >> 8) - (int)(ra >> 8)))
+            continue;
+          consider(ia, ib, agr[ia], agr[ib], ahead[ia], ahead[ib]);
         }
         drain();
       }
@@ -1475,34 +1523,62 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       }
       __syncthreads();
       const uint32_t nb = cnt;
-      // every (home run, neighbour item) pair within reach, in rounds:
-      // direct, or (offset-major, touching first) through the sub-cell map
-      const bool map = use_off && (uint32_t)na > nof;
-      const uint32_t total = map ? nof * nb : (uint32_t)na * nb;
-      for (uint32_t base = 0; base < total; base += ROUND) {
-        const uint32_t pe = min(total, base + (uint32_t)ROUND);
-        for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
-          uint32_t ia, ib;
-          if (map) {
-            const uint32_t oi = p / nb;
-            ib = p - oi * nb;
-            const uint32_t o = offs[oi], rb = brel[ib];
-            const int ax = rel_x(rb) + rel_x(o), ay = rel_y(rb) + rel_y(o),
-                      az = rel_z(rb) + rel_z(o);
+      // every (home run, neighbour item) pair within reach, in rounds
+      if (use_off && (uint32_t)na > nof) {
+        // items (offset, item), offset-major, through the map
+        const uint32_t total = nof * nb;
+        for (uint32_t base = 0; base < total; base += ROUND) {
+          const uint32_t pe = min(total, base + (uint32_t)ROUND);
+          for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
+            const uint32_t oi = p / nb, jb = p - oi * nb;
+            const uint32_t o = offs[oi], rb = brel[jb];
+            const int ax = rel_x(rb) + rel_x(o), ay = rel_y(rb) + rel_y(o), az = rel_z(rb) + rel_z(o);
             if (ax < 0 || ax >= mm[0] || ay < 0 || ay >= mm[1] || az < 0 || az >= mm[2]) continue;
-            ia = amap[(az * mm[1] + ay) * mm[0] + ax];
+            const uint32_t ia = amap[(az * mm[1] + ay) * mm[0] + ax];
             if (ia == kNoRun) continue;
-          } else {
-            ib = p / (uint32_t)na;
-            ia = p - ib * (uint32_t)na;
+            consider(ia, na + jb, agr[ia], bgr[jb], ahead[ia], bhead[jb]);
+          }
+          drain();
+        }
+      } else if (bmap) {
+        // items (row of the window, item), row-major, rows centre-out
+        const uint32_t total = (uint32_t)((2 * reach[2] + 1) * wy) * nb;
+        for (uint32_t base = 0; base < total; base += ROUND) {
+          const uint32_t pe = min(total, base + (uint32_t)ROUND);
+          for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
+            const uint32_t ri = p / nb, jb = p - ri * nb;
+            const int dz = zig((int)(ri / (uint32_t)wy)), dy = zig((int)(ri % (uint32_t)wy));
+            const uint32_t rb = brel[jb];
+            const int bx = rel_x(rb), ay = rel_y(rb) + dy, az = rel_z(rb) + dz;
+            if (az < 0 || az >= mm[2] || ay < 0 || ay >= mm[1] || !gap_ok(0, dy, dz)) continue;
+            const int x0 = max(bx - reach[0], 0), x1 = min(bx + reach[0], mm[0] - 1);
+            if (x0 > x1) continue;
+            const int rowb = (az * mm[1] + ay) * mm[0];
+            uint32_t bits = rowbits(rowb + x0, x1 - x0 + 1);
+            while (bits) {
+              const int b = __builtin_ctz(bits);
+              bits &= bits - 1;
+              if (!gap_ok(x0 + b - bx, dy, dz)) continue;
+              const uint32_t ia = amap[rowb + x0 + b];
+              consider(ia, na + jb, agr[ia], bgr[jb], ahead[ia], bhead[jb]);
+            }
+          }
+          drain();
+        }
+      } else {
+        const uint32_t total = (uint32_t)na * nb;
+        for (uint32_t base = 0; base < total; base += ROUND) {
+          const uint32_t pe = min(total, base + (uint32_t)ROUND);
+          for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
+            const uint32_t ib = p / (uint32_t)na, ia = p - ib * (uint32_t)na;
             const uint32_t ra = arel[ia], rb = brel[ib];
             if (!gap_ok(rel_x(rb) - (int)(ra & 15), rel_y(rb) - (int)((ra >> 4) & 15),
                         rel_z(rb) - (int)(ra >> 8)))
               continue;
+            consider(ia, na + ib, agr[ia], bgr[ib], ahead[ia], bhead[ib]);
           }
-          consider(ia, na + ib, agr[ia], bgr[ib], ahead[ia], bhead[ib]);
+          drain();
         }
-        drain();
       }
       // commit: a neighbour run joined in LDS joins its (home-cell) root's run
       for (uint32_t ib = threadIdx.x; ib < nb; ib += blockDim.x) {
@@ -2305,7 +2381,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   }
   if (ndense) {
     if ((rc = tb(15))) return rc;
-    k_link_dense<64, (int)kBigCell - 1, 128, 128, 128, 1024><<<std::min(ndense, 65536u), 64, 0, st>>>(
+    k_link_dense<64, (int)kBigCell - 1, 128, 128, 0, 1024><<<std::min(ndense, 65536u), 64, 0, st>>>(
         dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
         small + 4);
     FOF_HIP(hipGetLastError());

@@ -1185,19 +1185,18 @@ __device__ __forceinline__ uint16_t pack_home(int x, int y, int z) {
   return (uint16_t)(x | (y << 4) | (z << 8));
 }
 
-// Candidate enumeration in the dense kernels.  The 256-thread kernel lists
-// the sub-cell offsets within reach (packed 8 bits per axis, at most MAXOFF:
-// 5^3 - 1 = 124 at the usual reach of 2) and walks them offset-major,
-// touching offsets first, when the cell has enough runs; the one-wave kernel
-// (MAXOFF = 0) walks a bitmap of A's occupied sub-cells row by row when the
-// cell has more than kMapRuns runs.  Otherwise every run pair is tested.
-__device__ __forceinline__ uint32_t pack_off(int a, int b, int c) { return pack_rel(a, b, c); }
+// Cells with more runs than this enumerate candidates through a bitmap of
+// A's occupied sub-cells, one x-row of the window at a time; cells with fewer
+// test every run pair.  The 256-thread kernel walks the window twice: first
+// the sub-cells that touch the run's own (|offset| <= 1 on every axis), whose
+// pairs link most often, so that their merges prune the second walk over the
+// rest.
 constexpr int kMapRuns = 24;
 
 // 0, 1, -1, 2, -2, ...: window rows visited from the centre out.
 __device__ __forceinline__ int zig(int k) { return (k & 1) ? (k + 1) >> 1 : -(k >> 1); }
 
-template <int NT, int MAXRUNS, int MAXRUNSB, int BIGQ, int MAXOFF, int ROUND>
+template <int NT, int MAXRUNS, int MAXRUNSB, int BIGQ, int ROUND>
 __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ dense,
                                                     const uint32_t *__restrict__ ndense,
                                                     const float4 *__restrict__ pos4,
@@ -1222,14 +1221,12 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
   constexpr uint32_t kNoRun = (MAXRUNS < 256) ? 0xFFu : 0xFFFFu;
   __shared__ MapT amap[kMaxRuns];
   __shared__ uint32_t bigq[BIGQ];
-  constexpr bool OFFS = MAXOFF > 0;
-  constexpr int kOccW = OFFS ? 1 : kMaxRuns / 64 + 1;
-  __shared__ uint32_t offs[OFFS ? MAXOFF : 1];  // offsets within reach: touching ones first
-  __shared__ unsigned long long occ[kOccW];       // occupied sub-cells of A
+  constexpr int kOccW = kMaxRuns / 64 + 1;
+  __shared__ unsigned long long occ[kOccW];  // occupied sub-cells of A
   __shared__ uint32_t nbS[26], nbOff[27];
   __shared__ int nbLo[26][3], nbHi[26][3];
   __shared__ uint8_t nbDense[26];
-  __shared__ uint32_t cnt, qn, noff;
+  __shared__ uint32_t cnt, qn;
   // one-wave cells: run sizes in LDS (home runs fit a byte: < 256 particles)
   constexpr bool SIZES = (NT == 64);
   __shared__ uint8_t asz[SIZES ? MAXRUNS : 1];
@@ -1290,28 +1287,14 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
         nbLo[k][a] = lo, nbHi[k][a] = hi;
       }
     }
-    if (threadIdx.x == 0) cnt = 0, qn = 0, noff = 0;
-    if (!OFFS && threadIdx.x < kOccW) occ[threadIdx.x] = 0;
+    if (threadIdx.x == 0) cnt = 0, qn = 0;
+    if (threadIdx.x < kOccW) occ[threadIdx.x] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
       nbOff[0] = 0;
       for (int k = 0; k < 26; k++) nbOff[k + 1] += nbOff[k];
     }
-    // offsets within reach (gap test passed), in two passes so that the
-    // touching ones (|d| <= 1 on every axis) come first
-    const int wx = 2 * reach[0] + 1, wy = 2 * reach[1] + 1, wz = 2 * reach[2] + 1;
-    const bool use_off = OFFS && wx * wy * wz <= MAXOFF + 1;  // the zero offset is not listed
-    if (use_off) {
-      for (int pass = 0; pass < 2; pass++) {
-        for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
-          const int dx = t % wx - reach[0], dy = (t / wx) % wy - reach[1], dz = t / (wx * wy) - reach[2];
-          const bool touch = abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1;
-          if ((pass == 0) != touch || (dx == 0 && dy == 0 && dz == 0) || !gap_ok(dx, dy, dz)) continue;
-          offs[atomicAdd(&noff, 1u)] = pack_off(dx, dy, dz);
-        }
-        __syncthreads();
-      }
-    }
+    const int wy = 2 * reach[1] + 1;
     // the home cell's runs (one per occupied sub-cell) from its run table
     const uint32_t nruns = rtoff[sA];
     for (uint32_t t = threadIdx.x; t < nruns; t += blockDim.x) {
@@ -1326,7 +1309,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       lp[k] = k;
       const int bin = (rz * mm[1] + ry) * mm[0] + rx;
       amap[bin] = (MapT)k;
-      if (!OFFS) atomicOr(&occ[bin >> 6], 1ull << (bin & 63));
+      atomicOr(&occ[bin >> 6], 1ull << (bin & 63));
     }
     __syncthreads();
     const int na = (int)cnt;
@@ -1337,8 +1320,10 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       if (sh + len > 64) v |= occ[w + 1] << (64 - sh);
       return (uint32_t)(v & ((1ull << len) - 1));
     };
-    const uint32_t nof = use_off ? noff : 0u;
-    const bool bmap = !OFFS && na > kMapRuns;
+    const bool bmap = na > kMapRuns;
+    constexpr uint32_t kWalks = NT > 64 ? 2 : 1;
+    // bits of x in [c - 1, c + 1] for a row starting at x0, given s = c - 1 - x0
+    auto touchbits = [](int s) -> uint32_t { return s >= 0 ? 7u << s : 7u >> -s; };
     // a run pair (LDS slots sa, sb with heads ha, hb) not yet joined: small
     // pairs are tested here, big ones go to the wave queue
     auto test_pair = [&](uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb) {
@@ -1423,36 +1408,20 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
     };
     // (A, A), in rounds each followed by a drain, so that merges found early
     // prune the later rounds (a full queue falls back to testing in place)
-    if (use_off && (uint32_t)na > nof / 2) {
-      // items (offset, run), offset-major: the forward offsets through the map
-      const uint32_t total = nof * (uint32_t)na;
-      for (uint32_t base = 0; base < total; base += ROUND) {
-        const uint32_t pe = min(total, base + (uint32_t)ROUND);
-        for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
-          const uint32_t oi = p / (uint32_t)na, ia = p - oi * (uint32_t)na;
-          const uint32_t o = offs[oi];
-          const int dx = rel_x(o), dy = rel_y(o), dz = rel_z(o);
-          if (!(dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0))))) continue;  // forward
-          const uint32_t ra = arel[ia];
-          const int bx = (int)(ra & 15) + dx, by = (int)((ra >> 4) & 15) + dy, bz = (int)(ra >> 8) + dz;
-          if (bx < 0 || bx >= mm[0] || by < 0 || by >= mm[1] || bz >= mm[2]) continue;
-          const uint32_t ib = amap[(bz * mm[1] + by) * mm[0] + bx];
-          if (ib == kNoRun) continue;
-          const uint32_t s0 = min(ia, ib), s1 = max(ia, ib);
-          consider(s0, s1, agr[s0], agr[s1], ahead[s0], ahead[s1]);
-        }
-        drain();
-      }
-    } else if (bmap) {
+    if (bmap) {
       // items (row of the forward window, run), row-major, rows centre-out
       const uint32_t nrow = (uint32_t)((reach[2] + 1) * wy);
-      const uint32_t total = nrow * (uint32_t)na;
+      const uint32_t total = kWalks * nrow * (uint32_t)na;
       for (uint32_t base = 0; base < total; base += ROUND) {
         const uint32_t pe = min(total, base + (uint32_t)ROUND);
         for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
-          const uint32_t ri = p / (uint32_t)na, ia = p - ri * (uint32_t)na;
+          const uint32_t r2 = p / (uint32_t)na, ia = p - r2 * (uint32_t)na;
+          const bool first = kWalks == 2 && r2 < nrow;  // touching sub-cells only
+          const uint32_t ri = (kWalks == 2 && !first) ? r2 - nrow : r2;
           const int dz = (int)(ri / (uint32_t)wy), dy = zig((int)(ri % (uint32_t)wy));
           if (dz == 0 && dy < 0) continue;  // not forward
+          const bool trow = kWalks == 2 && dz <= 1 && dy >= -1 && dy <= 1;
+          if (first && !trow) continue;
           const uint32_t ra = arel[ia];
           const int ax = ra & 15, ay = (ra >> 4) & 15, az = ra >> 8;
           const int by = ay + dy, bz = az + dz;
@@ -1462,6 +1431,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
           if (x0 > x1) continue;
           const int rowb = (bz * mm[1] + by) * mm[0];
           uint32_t bits = rowbits(rowb + x0, x1 - x0 + 1);
+          if (trow) bits &= first ? touchbits(ax - 1 - x0) : ~touchbits(ax - 1 - x0);
           while (bits) {
             const int b = __builtin_ctz(bits);
             bits &= bits - 1;
@@ -1524,30 +1494,19 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       __syncthreads();
       const uint32_t nb = cnt;
       // every (home run, neighbour item) pair within reach, in rounds
-      if (use_off && (uint32_t)na > nof) {
-        // items (offset, item), offset-major, through the map
-        const uint32_t total = nof * nb;
-        for (uint32_t base = 0; base < total; base += ROUND) {
-          const uint32_t pe = min(total, base + (uint32_t)ROUND);
-          for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
-            const uint32_t oi = p / nb, jb = p - oi * nb;
-            const uint32_t o = offs[oi], rb = brel[jb];
-            const int ax = rel_x(rb) + rel_x(o), ay = rel_y(rb) + rel_y(o), az = rel_z(rb) + rel_z(o);
-            if (ax < 0 || ax >= mm[0] || ay < 0 || ay >= mm[1] || az < 0 || az >= mm[2]) continue;
-            const uint32_t ia = amap[(az * mm[1] + ay) * mm[0] + ax];
-            if (ia == kNoRun) continue;
-            consider(ia, na + jb, agr[ia], bgr[jb], ahead[ia], bhead[jb]);
-          }
-          drain();
-        }
-      } else if (bmap) {
+      if (bmap) {
         // items (row of the window, item), row-major, rows centre-out
-        const uint32_t total = (uint32_t)((2 * reach[2] + 1) * wy) * nb;
+        const uint32_t nrow = (uint32_t)((2 * reach[2] + 1) * wy);
+        const uint32_t total = kWalks * nrow * nb;
         for (uint32_t base = 0; base < total; base += ROUND) {
           const uint32_t pe = min(total, base + (uint32_t)ROUND);
           for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
-            const uint32_t ri = p / nb, jb = p - ri * nb;
+            const uint32_t r2 = p / nb, jb = p - r2 * nb;
+            const bool first = kWalks == 2 && r2 < nrow;
+            const uint32_t ri = (kWalks == 2 && !first) ? r2 - nrow : r2;
             const int dz = zig((int)(ri / (uint32_t)wy)), dy = zig((int)(ri % (uint32_t)wy));
+            const bool trow = kWalks == 2 && dz >= -1 && dz <= 1 && dy >= -1 && dy <= 1;
+            if (first && !trow) continue;
             const uint32_t rb = brel[jb];
             const int bx = rel_x(rb), ay = rel_y(rb) + dy, az = rel_z(rb) + dz;
             if (az < 0 || az >= mm[2] || ay < 0 || ay >= mm[1] || !gap_ok(0, dy, dz)) continue;
@@ -1555,6 +1514,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
             if (x0 > x1) continue;
             const int rowb = (az * mm[1] + ay) * mm[0];
             uint32_t bits = rowbits(rowb + x0, x1 - x0 + 1);
+            if (trow) bits &= first ? touchbits(bx - 1 - x0) : ~touchbits(bx - 1 - x0);
             while (bits) {
               const int b = __builtin_ctz(bits);
               bits &= bits - 1;
@@ -2373,7 +2333,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if ((rc = te(5))) return rc;
   if (nbig) {
     if ((rc = tb(6))) return rc;
-    k_link_dense<256, kMaxRuns, 448, 512, 512, 4096><<<std::min(nbig, 8192u), 256, 0, st>>>(
+    k_link_dense<256, kMaxRuns, 448, 512, 512><<<std::min(nbig, 8192u), 256, 0, st>>>(
         dbig, small + 7, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
         small + 4);
     FOF_HIP(hipGetLastError());
@@ -2381,7 +2341,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   }
   if (ndense) {
     if ((rc = tb(15))) return rc;
-    k_link_dense<64, (int)kBigCell - 1, 128, 128, 0, 1024><<<std::min(ndense, 65536u), 64, 0, st>>>(
+    k_link_dense<64, (int)kBigCell - 1, 128, 128, 512><<<std::min(ndense, 65536u), 64, 0, st>>>(
         dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
         small + 4);
     FOF_HIP(hipGetLastError());

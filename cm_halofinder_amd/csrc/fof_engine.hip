@@ -1199,6 +1199,18 @@ constexpr int kMapRuns = 24;
 // 0, 1, -1, 2, -2, ...: window rows visited from the centre out.
 __device__ __forceinline__ int zig(int k) { return (k & 1) ? (k + 1) >> 1 : -(k >> 1); }
 
+// Cell offset of neighbour k of the dense kernels' table: k < 13 the 13
+// forward cells of src_v2/FOF.c:122-153 ((1,0,0); (-1..1,1,0); (-1..1,-1..1,1)),
+// k >= 13 the same offsets negated.
+__device__ __forceinline__ void nb_offset(int k, int dd[3]) {
+  const int kf = k % 13, sg = k < 13 ? 1 : -1;
+  dd[0] = dd[1] = dd[2] = 0;
+  if (kf == 0) dd[0] = 1;
+  else if (kf < 4) dd[1] = 1, dd[0] = kf - 2;
+  else dd[2] = 1, dd[1] = (kf - 4) / 3 - 1, dd[0] = (kf - 4) % 3 - 1;
+  dd[0] *= sg, dd[1] *= sg, dd[2] *= sg;
+}
+
 template <int NT, int MAXRUNS, int MAXRUNSB, int BIGQ, int ROUND>
 __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ dense,
                                                     const uint32_t *__restrict__ ndense,
@@ -1219,15 +1231,14 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
   __shared__ uint16_t arel[MAXRUNS];
   __shared__ uint32_t bhead[MAXRUNSB], brel[MAXRUNSB], bgr[MAXRUNSB];
   __shared__ uint32_t lp[MAXRUNS + MAXRUNSB];
-  // sub-cell of A -> run slot (<= 12^3 sub-cells); bytes when slots fit
-  using MapT = typename std::conditional<(MAXRUNS < 256), uint8_t, uint16_t>::type;
-  constexpr uint32_t kNoRun = (MAXRUNS < 256) ? 0xFFu : 0xFFFFu;
-  __shared__ MapT amap[kMaxRuns];
   __shared__ uint32_t bigq[BIGQ];
+  // occupied sub-cells of A (<= 12^3) and the occupied count before each
+  // word: A's runs are listed in sub-cell order, so the run slot of an
+  // occupied sub-cell is its rank in the bitmap
   constexpr int kOccW = kMaxRuns / 64 + 1;
-  __shared__ unsigned long long occ[kOccW];  // occupied sub-cells of A
+  __shared__ unsigned long long occ[kOccW];
+  __shared__ uint16_t wpre[kOccW];
   __shared__ uint32_t nbS[26], nbOff[27];
-  __shared__ int nbLo[26][3], nbHi[26][3];
   __shared__ uint8_t nbDense[26];
   __shared__ uint32_t cnt, qn;
   // one-wave cells: run sizes in LDS (home runs fit a byte: < 256 particles)
@@ -1246,7 +1257,6 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
     const int ix = fine_x(f0) / mm[0], iy = fine_y(f0) / mm[1], iz = fine_z(f0) / mm[2];
     const Rule R = make_rule(V, ix, iy, iz);
     const int ox = ix * mm[0], oy = iy * mm[1], oz = iz * mm[2];
-    const int nbins = mm[0] * mm[1] * mm[2];  // <= 1728
     int reach[3];
     for (int a = 0; a < 3; a++)  // largest sub-cell offset whose gap can be within reach
       reach[a] = min((int)floor(sqrt(R.prune2 / V.s2[a])) + 1, mm[a]);
@@ -1258,19 +1268,15 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       const int gx = max(abs(dx) - 1, 0), gy = max(abs(dy) - 1, 0), gz = max(abs(dz) - 1, 0);
       return (float)(gx * gx) * gs0 + (float)(gy * gy) * gs1 + (float)(gz * gz) * gs2 <= gp2;
     };
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) amap[b] = (MapT)kNoRun;
     // neighbour table: the 13 forward cells of src_v2/FOF.c:122-153 and,
     // for a constant l on a dense grid, the 13 backward cells when sparse
     // (k_link_sparse leaves their pairs with dense cells to this kernel);
     // their items (runs of dense neighbours, particles of sparse ones) and
     // the window of sub-cells that can reach A
     if (threadIdx.x < 26) {
-      const int k = threadIdx.x, kf = k % 13, sg = k < 13 ? 1 : -1;
-      int dd[3] = {0, 0, 0};
-      if (kf == 0) dd[0] = 1;
-      else if (kf < 4) dd[1] = 1, dd[0] = kf - 2;
-      else dd[2] = 1, dd[1] = (kf - 4) / 3 - 1, dd[0] = (kf - 4) % 3 - 1;
-      dd[0] *= sg, dd[1] *= sg, dd[2] *= sg;
+      const int k = threadIdx.x;
+      int dd[3];
+      nb_offset(k, dd);
       const int bx = ix + dd[0], by = iy + dd[1], bz = iz + dd[2];
       uint32_t sB = 0, eB = 0;
       const bool want = k < 13 || (!G.sparse && V.vntab == 0);
@@ -1283,14 +1289,8 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       nbS[k] = sB;
       nbDense[k] = dn ? 1 : 0;
       nbOff[k + 1] = dn ? rtoff[sB] : eB - sB;  // runs of a dense cell, else particles
-      for (int a = 0; a < 3; a++) {
-        int lo = -1000, hi = 1000;
-        if (dd[a] == 1) lo = mm[a], hi = mm[a] - 1 + reach[a];
-        else if (dd[a] == -1) lo = -reach[a], hi = -1;
-        nbLo[k][a] = lo, nbHi[k][a] = hi;
-      }
     }
-    if (threadIdx.x == 0) cnt = 0, qn = 0;
+    if (threadIdx.x == 0) qn = 0;
     if (threadIdx.x < kOccW) occ[threadIdx.x] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1298,24 +1298,33 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
       for (int k = 0; k < 26; k++) nbOff[k + 1] += nbOff[k];
     }
     const int wy = 2 * reach[1] + 1;
-    // the home cell's runs (one per occupied sub-cell) from its run table
-    const uint32_t nruns = rtoff[sA];
-    for (uint32_t t = threadIdx.x; t < nruns; t += blockDim.x) {
-      const uint32_t j = rtab[sA + t];
+    // the home cell's runs (one per occupied sub-cell, in sub-cell order)
+    // from its run table; runs <= min(particles, sub-cells) <= MAXRUNS
+    const int na = (int)rtoff[sA];
+    for (uint32_t k = threadIdx.x; k < (uint32_t)na; k += blockDim.x) {
+      const uint32_t j = rtab[sA + k];
       const unsigned long long f = fine[j];
       const int rx = fine_x(f) - ox, ry = fine_y(f) - oy, rz = fine_z(f) - oz;
-      const uint32_t k = atomicAdd(&cnt, 1u);  // runs <= min(particles, sub-cells) <= MAXRUNS
       ahead[k] = j;
       arel[k] = pack_home(rx, ry, rz);
       if (SIZES) asz[k] = (uint8_t)(run_end[j] - j);
       agr[k] = uf_find(par, j);  // snapshot of the global root
       lp[k] = k;
       const int bin = (rz * mm[1] + ry) * mm[0] + rx;
-      amap[bin] = (MapT)k;
       atomicOr(&occ[bin >> 6], 1ull << (bin & 63));
     }
     __syncthreads();
-    const int na = (int)cnt;
+    if (threadIdx.x < kOccW) {
+      uint32_t n = 0;
+      for (uint32_t w = 0; w < threadIdx.x; w++) n += __popcll(occ[w]);
+      wpre[threadIdx.x] = (uint16_t)n;
+    }
+    __syncthreads();
+    // run slot of the first occupied sub-cell at or after bin
+    auto rank = [&](int bin) -> uint32_t {
+      const int w = bin >> 6;
+      return wpre[w] + (uint32_t)__popcll(occ[w] & ((1ull << (bin & 63)) - 1));
+    };
     // bits of the occupied sub-cells [bin0, bin0 + len) of A (len <= 25)
     auto rowbits = [&](int bin0, int len) -> uint32_t {
       const int w = bin0 >> 6, sh = bin0 & 63;
@@ -1432,14 +1441,15 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
           const int x0 = (dz == 0 && dy == 0) ? ax + 1 : max(ax - reach[0], 0);
           const int x1 = min(ax + reach[0], mm[0] - 1);
           if (x0 > x1) continue;
-          const int rowb = (bz * mm[1] + by) * mm[0];
-          uint32_t bits = rowbits(rowb + x0, x1 - x0 + 1);
+          const int bin0 = (bz * mm[1] + by) * mm[0] + x0;
+          const uint32_t all = rowbits(bin0, x1 - x0 + 1), r0 = rank(bin0);
+          uint32_t bits = all;
           if (trow) bits &= first ? touchbits(ax - 1 - x0) : ~touchbits(ax - 1 - x0);
           while (bits) {
             const int b = __builtin_ctz(bits);
             bits &= bits - 1;
             if (!gap_ok(x0 + b - ax, dy, dz)) continue;
-            const uint32_t ib = amap[rowb + x0 + b];
+            const uint32_t ib = r0 + __popc(all & ((1u << b) - 1));
             const uint32_t s0 = min(ia, ib), s1 = max(ia, ib);
             consider(s0, s1, agr[s0], agr[s1], ahead[s0], ahead[s1]);
           }
@@ -1483,10 +1493,15 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
           const CellPos cj = cell_pos(V, pj.x, pj.y, pj.z);
           fx = cj.fx, fy = cj.fy, fz = cj.fz;
         }
-        const int rx = fx - ox, ry = fy - oy, rz = fz - oz;
-        if (rx < nbLo[q][0] || rx > nbHi[q][0] || ry < nbLo[q][1] || ry > nbHi[q][1] ||
-            rz < nbLo[q][2] || rz > nbHi[q][2])
-          continue;
+        // within reach of A: at most reach sub-cells beyond A's face
+        const int rr[3] = {fx - ox, fy - oy, fz - oz};
+        int dd[3];
+        nb_offset(q, dd);
+        bool far = false;
+        for (int a = 0; a < 3; a++)
+          far |= (dd[a] == 1 && rr[a] > mm[a] - 1 + reach[a]) || (dd[a] == -1 && rr[a] < -reach[a]);
+        if (far) continue;
+        const int rx = rr[0], ry = rr[1], rz = rr[2];
         const uint32_t k = atomicAdd(&cnt, 1u);  // < MAXRUNSB: one per item at most
         bhead[k] = j;
         brel[k] = pack_rel(rx, ry, rz);
@@ -1515,14 +1530,15 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
             if (az < 0 || az >= mm[2] || ay < 0 || ay >= mm[1] || !gap_ok(0, dy, dz)) continue;
             const int x0 = max(bx - reach[0], 0), x1 = min(bx + reach[0], mm[0] - 1);
             if (x0 > x1) continue;
-            const int rowb = (az * mm[1] + ay) * mm[0];
-            uint32_t bits = rowbits(rowb + x0, x1 - x0 + 1);
+            const int bin0 = (az * mm[1] + ay) * mm[0] + x0;
+            const uint32_t all = rowbits(bin0, x1 - x0 + 1), r0 = rank(bin0);
+            uint32_t bits = all;
             if (trow) bits &= first ? touchbits(bx - 1 - x0) : ~touchbits(bx - 1 - x0);
             while (bits) {
               const int b = __builtin_ctz(bits);
               bits &= bits - 1;
               if (!gap_ok(x0 + b - bx, dy, dz)) continue;
-              const uint32_t ia = amap[rowb + x0 + b];
+              const uint32_t ia = r0 + __popc(all & ((1u << b) - 1));
               consider(ia, na + jb, agr[ia], bgr[jb], ahead[ia], bhead[jb]);
             }
           }

@@ -1148,8 +1148,9 @@ __device__ __forceinline__ void lunion(volatile uint32_t *lp, uint32_t a, uint32
   }
 }
 
-// Any linking pair between two runs (both cliques)?  Early exit.  The
-// partner positions are loaded eight at a time so that the loads overlap.
+// Any linking pair between two runs (both cliques)?  Early exit.  Partner
+// positions are loaded two at a time: the dense kernels run at 6-8 waves per
+// SIMD, where wider batches cost more in registers than they hide latency.
 __device__ __forceinline__ bool runs_link(const float4 *__restrict__ pos4, uint32_t ha,
                                           uint32_t ea, uint32_t hb, uint32_t eb, const Rule &R,
                                           uint32_t &tests, uint32_t &rechecks) {
@@ -1161,14 +1162,14 @@ __device__ __forceinline__ bool runs_link(const float4 *__restrict__ pos4, uint3
   for (uint32_t i = ha; i < ea; i++) {
     const float4 pi = pos4[i];
     uint32_t j = hb;
-    for (; j + 8 <= eb; j += 8) {
-      float4 q[8];
+    for (; j + 2 <= eb; j += 2) {
+      float4 q[2];
 #pragma unroll
-      for (int k = 0; k < 8; k++) q[k] = pos4[j + k];
+      for (int k = 0; k < 2; k++) q[k] = pos4[j + k];
       bool any = false;
 #pragma unroll
-      for (int k = 0; k < 8; k++) any |= pair_linked(pi, q[k], R, rechecks);
-      tests += 8;
+      for (int k = 0; k < 2; k++) any |= pair_linked(pi, q[k], R, rechecks);
+      tests += 2;
       if (any) return true;
     }
     for (; j < eb; j++) {
@@ -1211,8 +1212,16 @@ __device__ __forceinline__ void nb_offset(int k, int dd[3]) {
   dd[0] *= sg, dd[1] *= sg, dd[2] *= sg;
 }
 
-template <int NT, int MAXRUNS, int MAXRUNSB, int BIGQ, int ROUND>
-__global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ dense,
+// Bytes of dynamic LDS for ra home-run slots and MAXRUNSB neighbour slots:
+// ahead, agr, lp (ra + MAXRUNSB), arel (u16), asz (u8, one-wave kernel only).
+constexpr size_t dense_lds(uint32_t ra, uint32_t maxrunsb, bool sizes) {
+  return 4 * ((size_t)3 * ra + maxrunsb) + 2 * (size_t)ra + (sizes ? (size_t)ra : 0) + 8;
+}
+
+// WPE: waves per SIMD to fit (the register budget; LDS per workgroup is
+// sized at launch to the runs a cell of the batch can have).
+template <int NT, int WPE, int MAXRUNS, int MAXRUNSB, int BIGQ, int ROUND>
+__global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restrict__ dense,
                                                     const uint32_t *__restrict__ ndense,
                                                     const float4 *__restrict__ pos4,
                                                     const Grid G,
@@ -1226,11 +1235,15 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
                                                     const uint32_t *__restrict__ vbeg,
                                                     int nvol,
                                                     unsigned long long *__restrict__ ctr,
-                                                    uint32_t *__restrict__ overflow) {
-  __shared__ uint32_t ahead[MAXRUNS], agr[MAXRUNS];
-  __shared__ uint16_t arel[MAXRUNS];
+                                                    uint32_t *__restrict__ overflow,
+                                                    uint32_t ra) {
+  // per-run slots of the home cell (ra <= MAXRUNS of them: the most
+  // sub-cells of a cell in the batch), in dynamic LDS
+  extern __shared__ uint32_t dyn[];
+  uint32_t *const ahead = dyn, *const agr = dyn + ra, *const lp = dyn + 2 * ra;
+  uint16_t *const arel = reinterpret_cast<uint16_t *>(dyn + 3 * ra + MAXRUNSB);
+  uint8_t *const asz = reinterpret_cast<uint8_t *>(arel + ra);
   __shared__ uint32_t bhead[MAXRUNSB], brel[MAXRUNSB], bgr[MAXRUNSB];
-  __shared__ uint32_t lp[MAXRUNS + MAXRUNSB];
   __shared__ uint32_t bigq[BIGQ];
   // occupied sub-cells of A (<= 12^3) and the occupied count before each
   // word: A's runs are listed in sub-cell order, so the run slot of an
@@ -1243,7 +1256,6 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
   __shared__ uint32_t cnt, qn;
   // one-wave cells: run sizes in LDS (home runs fit a byte: < 256 particles)
   constexpr bool SIZES = (NT == 64);
-  __shared__ uint8_t asz[SIZES ? MAXRUNS : 1];
   __shared__ uint16_t bsz[SIZES ? MAXRUNSB : 1];
   uint32_t tests = 0, rechecks = 0;
   const uint32_t lane = threadIdx.x & 63;
@@ -1299,7 +1311,7 @@ __global__ __launch_bounds__(NT) void k_link_dense(const uint32_t *__restrict__ 
     }
     const int wy = 2 * reach[1] + 1;
     // the home cell's runs (one per occupied sub-cell, in sub-cell order)
-    // from its run table; runs <= min(particles, sub-cells) <= MAXRUNS
+    // from its run table; runs <= min(particles, sub-cells) <= ra
     const int na = (int)rtoff[sA];
     for (uint32_t k = threadIdx.x; k < (uint32_t)na; k += blockDim.x) {
       const uint32_t j = rtab[sA + k];
@@ -2350,19 +2362,30 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   k_link_sparse<<<gN, 256, 0, st>>>(NN, pos4, G, runend, fine, par, dV, dVB, (int)nvol, T, ctr);
   FOF_HIP(hipGetLastError());
   if ((rc = te(5))) return rc;
+  // the most sub-cells (so runs) a dense cell of the batch can have
+  uint32_t maxbins = 1;
+  for (uint32_t v = 0; v < nvol; v++)
+    if (hv[v].d.m[0] > 0)
+      maxbins = std::max(maxbins, (uint32_t)(hv[v].d.m[0] * hv[v].d.m[1] * hv[v].d.m[2]));
   if (nbig) {
     if ((rc = tb(6))) return rc;
-    k_link_dense<256, kMaxRuns, 448, 512, 512><<<std::min(nbig, 8192u), 256, 0, st>>>(
-        dbig, small + 7, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
-        small + 4);
+    constexpr int kRB = 448;
+    const uint32_t ra = std::min(maxbins, (uint32_t)kMaxRuns);
+    k_link_dense<256, 8, kMaxRuns, kRB, 512, 512>
+        <<<std::min(nbig, 8192u), 256, dense_lds(ra, kRB, false), st>>>(
+            dbig, small + 7, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol,
+            ctr, small + 4, ra);
     FOF_HIP(hipGetLastError());
     if ((rc = te(6))) return rc;
   }
   if (ndense) {
     if ((rc = tb(15))) return rc;
-    k_link_dense<64, (int)kBigCell - 1, 128, 128, 512><<<std::min(ndense, 65536u), 64, 0, st>>>(
-        dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol, ctr,
-        small + 4);
+    constexpr int kRB = 64;
+    const uint32_t ra = std::min(maxbins, kBigCell - 1);
+    k_link_dense<64, 6, (int)kBigCell - 1, kRB, 64, 512>
+        <<<std::min(ndense, 65536u), 64, dense_lds(ra, kRB, true), st>>>(
+            dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol,
+            ctr, small + 4, ra);
     FOF_HIP(hipGetLastError());
     if ((rc = te(15))) return rc;
   }

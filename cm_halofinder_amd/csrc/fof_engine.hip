@@ -386,24 +386,25 @@ __global__ __launch_bounds__(256) void k_pa_count(const uint8_t *__restrict__ ba
 // so that most output lines are complete before L2 evicts them (1024 was
 // 10% slower on the 1024^3 box: partial lines written twice).
 constexpr int kSub = 2048;
+constexpr int kPartNT = 256;             // threads of the partition scatter kernels
+constexpr int kPer = kSub / kPartNT;     // elements a thread holds per sub-tile
 
 template <int NB>
-__device__ __forceinline__ void staged_part(uint32_t nsub, const uint32_t (&kk)[kSub / 256],
-                                            const float4 (&pp)[kSub / 256],
-                                            const uint32_t (&bk)[kSub / 256], uint32_t nbk,
+__device__ __forceinline__ void staged_part(uint32_t nsub, const uint32_t (&kk)[kPer],
+                                            const float4 (&pp)[kPer],
+                                            const uint32_t (&bk)[kPer], uint32_t nbk,
                                             uint32_t *cur, uint32_t *lcnt, uint32_t *loff,
                                             uint32_t *skey, float4 *spos, uint32_t *sbk,
                                             uint32_t *ws, uint32_t *__restrict__ okey,
                                             float4 *__restrict__ opos) {
-  constexpr int kPer = kSub / 256;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t rank[kPer];
 #pragma unroll
   for (int k = 0; k < kPer; k++)
-    if (threadIdx.x + k * 256 < nsub) rank[k] = atomicAdd(&lcnt[bk[k]], 1u);
+    if (threadIdx.x + k * kPartNT < nsub) rank[k] = atomicAdd(&lcnt[bk[k]], 1u);
   __syncthreads();
-  // exclusive scan of lcnt over the buckets (NB / 256 per thread)
-  constexpr int kB = NB / 256;
+  // exclusive scan of lcnt over the buckets (kB consecutive per thread)
+  constexpr int kB = (NB + kPartNT - 1) / kPartNT;
   uint32_t c[kB], loc = 0;
 #pragma unroll
   for (int q = 0; q < kB; q++) {
@@ -430,14 +431,14 @@ __device__ __forceinline__ void staged_part(uint32_t nsub, const uint32_t (&kk)[
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kPer; k++)
-    if (threadIdx.x + k * 256 < nsub) {
+    if (threadIdx.x + k * kPartNT < nsub) {
       const uint32_t j = loff[bk[k]] + rank[k];
       skey[j] = kk[k];
       spos[j] = pp[k];
       sbk[j] = bk[k];
     }
   __syncthreads();
-  for (uint32_t j = threadIdx.x; j < nsub; j += 256) {
+  for (uint32_t j = threadIdx.x; j < nsub; j += kPartNT) {
     const uint32_t b = sbk[j];
     const uint32_t g = cur[b] + (j - loff[b]);
     okey[g] = skey[j];
@@ -453,7 +454,7 @@ __device__ __forceinline__ void staged_part(uint32_t nsub, const uint32_t (&kk)[
 }
 
 // A, scatter: keys and {x,y,z,index} to the tile's slots of each coarse bucket.
-__global__ __launch_bounds__(256) void k_pa_scatter(const uint8_t *__restrict__ base,
+__global__ __launch_bounds__(kPartNT) void k_pa_scatter(const uint8_t *__restrict__ base,
                                                     uint32_t stride, uint32_t offpos, uint32_t N,
                                                     uint32_t Bc, uint32_t ntiles,
                                                     const uint32_t *__restrict__ key,
@@ -463,7 +464,7 @@ __global__ __launch_bounds__(256) void k_pa_scatter(const uint8_t *__restrict__ 
   __shared__ uint32_t cur[kMaxCoarse], lcnt[kMaxCoarse], loff[kMaxCoarse];
   __shared__ uint32_t skey[kSub], sbk[kSub];
   __shared__ float4 spos[kSub];
-  __shared__ uint32_t ws[4];
+  __shared__ uint32_t ws[kPartNT / 64];
   const uint32_t tile = blockIdx.x;
   for (uint32_t b = threadIdx.x; b < kMaxCoarse; b += blockDim.x) {
     cur[b] = b < Bc ? off[(uint64_t)b * ntiles + tile] : 0u;
@@ -473,11 +474,11 @@ __global__ __launch_bounds__(256) void k_pa_scatter(const uint8_t *__restrict__ 
   const uint32_t p0 = tile * kPartTile, p1 = min(N, p0 + kPartTile);
   for (uint32_t q0 = p0; q0 < p1; q0 += kSub) {
     const uint32_t nsub = min((uint32_t)kSub, p1 - q0);
-    uint32_t kk[kSub / 256], bk[kSub / 256];
-    float4 pp[kSub / 256];
+    uint32_t kk[kPer], bk[kPer];
+    float4 pp[kPer];
 #pragma unroll
-    for (int k = 0; k < kSub / 256; k++) {
-      const uint32_t i = threadIdx.x + k * 256;
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = threadIdx.x + k * kPartNT;
       if (i < nsub) {
         const uint32_t p = q0 + i;
         const uint32_t c = key[p];
@@ -528,7 +529,7 @@ __global__ __launch_bounds__(256) void k_pb_count(const uint32_t *__restrict__ t
   hist[256ull * t0[c] + (uint64_t)threadIdx.x * nt + k] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(256) void k_pb_scatter(const uint32_t *__restrict__ tinfo,
+__global__ __launch_bounds__(kPartNT) void k_pb_scatter(const uint32_t *__restrict__ tinfo,
                                                     const uint32_t *__restrict__ t0,
                                                     const uint32_t *__restrict__ cstart,
                                                     const uint32_t *__restrict__ off,
@@ -539,19 +540,21 @@ __global__ __launch_bounds__(256) void k_pb_scatter(const uint32_t *__restrict__
   __shared__ uint32_t cur[256], lcnt[256], loff[256];
   __shared__ uint32_t skey[kSub], sbk[kSub];
   __shared__ float4 spos[kSub];
-  __shared__ uint32_t ws[4];
+  __shared__ uint32_t ws[kPartNT / 64];
   uint32_t c, i0, i1, nt, k;
   pb_tile(blockIdx.x, tinfo, t0, cstart, c, i0, i1, nt, k);
-  cur[threadIdx.x] = off[256ull * t0[c] + (uint64_t)threadIdx.x * nt + k];
-  lcnt[threadIdx.x] = 0;
+  for (uint32_t b = threadIdx.x; b < 256; b += kPartNT) {
+    cur[b] = off[256ull * t0[c] + (uint64_t)b * nt + k];
+    lcnt[b] = 0;
+  }
   __syncthreads();
   for (uint32_t q0 = i0; q0 < i1; q0 += kSub) {
     const uint32_t nsub = min((uint32_t)kSub, i1 - q0);
-    uint32_t kk[kSub / 256], bk[kSub / 256];
-    float4 pp[kSub / 256];
+    uint32_t kk[kPer], bk[kPer];
+    float4 pp[kPer];
 #pragma unroll
-    for (int q = 0; q < kSub / 256; q++) {
-      const uint32_t i = threadIdx.x + q * 256;
+    for (int q = 0; q < kPer; q++) {
+      const uint32_t i = threadIdx.x + q * kPartNT;
       if (i < nsub) {
         kk[q] = ikey[q0 + i];
         bk[q] = (kk[q] >> kFineBits) & 255u;
@@ -939,7 +942,7 @@ __global__ __launch_bounds__(NT) void k_subsort(const uint32_t *__restrict__ den
 // ---------------------------------------------------------------------------
 // 5: sparse cells -- one thread per particle.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *__restrict__ pos4,
+__global__ __launch_bounds__(256, 8) void k_link_sparse(uint32_t N, const float4 *__restrict__ pos4,
                                                      const Grid G,
                                                      const uint32_t *__restrict__ run_end,
                                                      const unsigned long long *__restrict__ fine,
@@ -965,7 +968,7 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
       // Constant l on a dense grid: k_link_dense links every pair that has a
       // particle in a dense cell (its forward cells, and its backward cells
       // that are sparse).  Here: the sparse cells among the own cell (after
-      // x) and the 13 forward cells, partners loaded eight at a time.
+      // x) and the 13 forward cells, partners loaded four at a time.
       // Row segments: own row (ix, ix+1); (iy+1, iz) and (iy-1..iy+1, iz+1)
       // over ix-1..ix+1.  b[r][0..3] are the segment's cell boundaries.
       uint32_t b[5][4];
@@ -995,16 +998,16 @@ __global__ __launch_bounds__(256) void k_link_sparse(uint32_t N, const float4 *_
           if (d1 && j >= b[r][1] && j < b[r][2]) j = b[r][2];
           if (d2 && j >= b[r][2]) j = b[r][3];
           if (j >= b[r][3]) break;
-          uint32_t je = min(j + 8u, b[r][3]);
+          uint32_t je = min(j + 4u, b[r][3]);
           if (d1 && j < b[r][1]) je = min(je, b[r][1]);
           if (d2 && j < b[r][2]) je = min(je, b[r][2]);
           const uint32_t n = je - j;
-          float4 q[8];
+          float4 q[4];
 #pragma unroll
-          for (int k = 0; k < 8; k++) q[k] = ((uint32_t)k < n) ? pos4[j + k] : px;
+          for (int k = 0; k < 4; k++) q[k] = ((uint32_t)k < n) ? pos4[j + k] : px;
           uint32_t hit = 0;
 #pragma unroll
-          for (int k = 0; k < 8; k++)
+          for (int k = 0; k < 4; k++)
             hit |= ((uint32_t)k < n && pair_linked(px, q[k], R, rechecks)) ? (1u << k) : 0u;
           tests += n;
           while (hit) {
@@ -2267,13 +2270,13 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     FOF_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
     if ((rc = te(1))) return rc;
     if ((rc = tb(2))) return rc;
-    k_pa_scatter<<<ntA, 256, 0, st>>>(dbase, stride, offpos, NN, Bc, ntA, key, offA, k1, p1);
+    k_pa_scatter<<<ntA, kPartNT, 0, st>>>(dbase, stride, offpos, NN, Bc, ntA, key, offA, k1, p1);
     FOF_HIP(hipGetLastError());
     if (ntB) {
       k_pb_count<<<ntB, 256, 0, st>>>(tinfo, t0, cst, k1, histB);
       FOF_HIP(hipGetLastError());
       if ((rc = scan_u32(histB, offB, 256ull * ntB, W.tmp.ptr, st))) return rc;
-      k_pb_scatter<<<ntB, 256, 0, st>>>(tinfo, t0, cst, offB, k1, p1, k2, p2);
+      k_pb_scatter<<<ntB, kPartNT, 0, st>>>(tinfo, t0, cst, offB, k1, p1, k2, p2);
       FOF_HIP(hipGetLastError());
     }
     k_pb_starts<<<grid_for(F + 1), 256, 0, st>>>(F, t0, cst, offB, NN, fst);

@@ -589,8 +589,13 @@ __device__ __forceinline__ int vol_of_cell(const VolDev *__restrict__ vols, int 
   return lo;
 }
 
-// C: one workgroup per fine bucket, all counting in LDS.
-constexpr int kPcThreads = 512;
+// C: one workgroup per fine bucket, all counting in LDS.  The dense-cell
+// lists are gathered per workgroup in LDS and appended with one atomic per
+// list (per-wave appends to the two global counters were a third of the
+// kernel's time); a workgroup whose lists outgrow LDS appends the rest per
+// wave.
+constexpr int kPcThreads = 1024;
+constexpr uint32_t kPcListS = 1536, kPcListB = 512;
 
 __global__ __launch_bounds__(kPcThreads) void k_pc_sort(uint64_t C, const uint32_t *__restrict__ fstart,
                                                  const uint32_t *__restrict__ ikey,
@@ -607,16 +612,20 @@ __global__ __launch_bounds__(kPcThreads) void k_pc_sort(uint64_t C, const uint32
   __shared__ uint32_t lc[kFineCells + kFineCells / 64 + 1];
   auto P = [](uint32_t e) { return e + (e >> 6); };
   __shared__ uint32_t ws[kPcThreads / 64];
+  __shared__ uint32_t lsml[kPcListS], lbig[kPcListB];
+  // entries reserved, first reservation that did not fit, global bases
+  __shared__ uint32_t nres[2], nfit[2], gbase[2];
   const uint32_t f = blockIdx.x;
   const uint32_t q0 = fstart[f], q1 = fstart[f + 1];
   const uint64_t c0 = (uint64_t)f << kFineBits;
   const uint32_t nc = (uint32_t)min((uint64_t)kFineCells, C - c0);
   for (uint32_t e = threadIdx.x; e < kFineCells + kFineCells / 64 + 1; e += blockDim.x) lc[e] = 0;
+  if (threadIdx.x < 2) nres[threadIdx.x] = 0, nfit[threadIdx.x] = threadIdx.x ? kPcListB : kPcListS;
   __syncthreads();
   for (uint32_t i = q0 + threadIdx.x; i < q1; i += blockDim.x)
     atomicAdd(&lc[P(ikey[i] & (kFineCells - 1))], 1u);
   __syncthreads();
-  // exclusive scan of the 2^14 counters: 64 consecutive per thread
+  // exclusive scan of the 2^14 counters: kPer consecutive per thread
   constexpr uint32_t kPer = kFineCells / kPcThreads;
   const uint32_t e0 = threadIdx.x * kPer;
   uint32_t loc = 0;
@@ -639,7 +648,27 @@ __global__ __launch_bounds__(kPcThreads) void k_pc_sort(uint64_t C, const uint32
   }
   if (threadIdx.x == kPcThreads - 1) lc[P(kFineCells)] = pre;  // bucket total
   __syncthreads();
-  // cell starts (coalesced) and the dense-cell lists (one append per wave)
+  // one wave's dense cells of one kind: into the LDS list while it fits
+  // (the fitting reservations are a prefix of the list), else appended
+  auto append = [&](int kind, bool mine, uint32_t cell) {
+    const unsigned long long bal = __ballot(mine);
+    if (!bal) return;
+    const uint32_t cnt = (uint32_t)__popcll(bal), below = (uint32_t)__popcll(bal & ((1ull << lane) - 1));
+    const uint32_t cap = kind ? kPcListB : kPcListS;
+    uint32_t base = 0;
+    if (lane == 0) {
+      base = atomicAdd(&nres[kind], cnt);
+      if (base + cnt > cap) {
+        atomicMin(&nfit[kind], base);
+        base = 0x80000000u | atomicAdd(kind ? nbig : ndense, cnt);
+      }
+    }
+    base = __shfl(base, 0, 64);
+    if (!mine) return;
+    if (base & 0x80000000u) (kind ? dbig : dense)[(base & 0x7FFFFFFFu) + below] = cell;
+    else (kind ? lbig : lsml)[base + below] = cell;
+  };
+  // cell starts (coalesced) and the dense-cell lists
   for (uint32_t e0 = threadIdx.x & ~63u; e0 < nc; e0 += blockDim.x) {
     const uint32_t e = e0 + lane;
     bool sml = false, big = false;
@@ -651,23 +680,19 @@ __global__ __launch_bounds__(kPcThreads) void k_pc_sort(uint64_t C, const uint32
         sml = !big;
       }
     }
-    const unsigned long long bs = __ballot(sml), bb = __ballot(big);
-    const unsigned long long below = (1ull << lane) - 1;
-    if (bs) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(ndense, (uint32_t)__popcll(bs));
-      base = __shfl(base, 0, 64);
-      if (sml) dense[base + __popcll(bs & below)] = (uint32_t)(c0 + e);
-    }
-    if (bb) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(nbig, (uint32_t)__popcll(bb));
-      base = __shfl(base, 0, 64);
-      if (big) dbig[base + __popcll(bb & below)] = (uint32_t)(c0 + e);
-    }
+    append(0, sml, (uint32_t)(c0 + e));
+    append(1, big, (uint32_t)(c0 + e));
   }
   if (c0 + nc == C && threadIdx.x == 0) start[C] = q1;
   __syncthreads();
+  if (threadIdx.x < 2) {
+    const uint32_t k = threadIdx.x, m = min(nres[k], nfit[k]);
+    nfit[k] = m;
+    gbase[k] = m ? atomicAdd(k ? nbig : ndense, m) : 0u;
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < nfit[0]; k += blockDim.x) dense[gbase[0] + k] = lsml[k];
+  for (uint32_t k = threadIdx.x; k < nfit[1]; k += blockDim.x) dbig[gbase[1] + k] = lbig[k];
   // scatter with LDS cursors
   for (uint32_t i = q0 + threadIdx.x; i < q1; i += blockDim.x) {
     const uint32_t slot = q0 + atomicAdd(&lc[P(ikey[i] & (kFineCells - 1))], 1u);

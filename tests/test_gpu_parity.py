@@ -89,6 +89,24 @@ def test_random_dense_blobs(ctx):
     check_against_oracle(ctx, P, dom)
 
 
+def test_many_big_cells_in_one_bucket(ctx):
+    # 640 cells of 260 particles each, all among the first 2^14 cell ids:
+    # one sort bucket lists more big dense cells (>= 256 particles) than its
+    # LDS list holds, so part of the list is appended directly
+    rng = np.random.default_rng(21)
+    nx, ny, per, lc = 32, 20, 260, 0.4
+    ix, iy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    corner = np.stack([ix.ravel() * lc, iy.ravel() * lc, np.zeros(nx * ny)], 1)
+    pos = np.repeat(corner, per, 0) + rng.random((nx * ny * per, 3)) * lc * 0.999
+    n = len(pos)
+    P = np.zeros(n, L.PART_DTYPE)
+    P["Pos"] = pos.astype(np.float32)
+    P["Vel"] = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    dom = L.domain([0] * 3, [nx * lc, ny * lc, lc], [0] * 3, [nx * lc, ny * lc, lc], [lc] * 3, 0.08, 5)
+    lab, st = check_against_oracle(ctx, P, dom)
+    assert st.ndense >= nx * ny
+
+
 def test_exact_ties(ctx):
     # d^2 == l^2 exactly links (src_v2/FOF.c:357); one ulp further does not
     l = 25.0 / 128.0

@@ -303,6 +303,16 @@ __device__ __forceinline__ uint32_t uf_join(uint32_t *par, uint32_t rx, uint32_t
   }
 }
 
+// Statistics counters (pair tests, FP64 rechecks, groups, owned halos) are
+// spread over kCtrSlots copies, one 128-byte line each, picked by workgroup:
+// a million waves adding to one address serialise at one L2 channel.  The
+// host sums the copies.
+constexpr int kCtrSlots = 64;
+constexpr int kCtrStride = 16;  // u64 per slot
+__device__ __forceinline__ void ctr_add(unsigned long long *ctr, int k, unsigned long long v) {
+  atomicAdd(ctr + (blockIdx.x & (kCtrSlots - 1)) * kCtrStride + k, v);
+}
+
 __device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t tests,
                                              uint32_t rechecks) {
   uint64_t t = tests, r = rechecks;
@@ -312,8 +322,8 @@ __device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t t
     r += __shfl_down(r, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
-    if (t) atomicAdd(ctr + 0, (unsigned long long)t);
-    if (r) atomicAdd(ctr + 1, (unsigned long long)r);
+    if (t) ctr_add(ctr, 0, t);
+    if (r) ctr_add(ctr, 1, r);
   }
 }
 
@@ -1667,7 +1677,7 @@ __global__ __launch_bounds__(256) void k_flatten(uint32_t N, uint32_t *__restric
   uint64_t t = roots;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
-  if ((threadIdx.x & 63) == 0 && t) atomicAdd(ctr + 2, (unsigned long long)t);
+  if ((threadIdx.x & 63) == 0 && t) ctr_add(ctr, 2, t);
 }
 
 // label[orig] = group minimum index (global), with kFlag when the group is
@@ -1830,7 +1840,7 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
   uint64_t t = owned_cnt;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
-  if ((threadIdx.x & 63) == 0 && t) atomicAdd(ctr + 3, (unsigned long long)t);
+  if ((threadIdx.x & 63) == 0 && t) ctr_add(ctr, 3, t);
 }
 
 // halo_offsets[v] = first halo whose global head >= vbeg[v].
@@ -2120,7 +2130,8 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if ((rc = ensure(W.tmp, tmpb))) return rc;
   if ((rc = ensure(W.vols, nvol * sizeof(VolDev) + vt_entries * sizeof(double) + 256))) return rc;
   if ((rc = ensure(W.vbeg, (nvol + 1) * 4 + 64))) return rc;
-  if ((rc = ensure(W.small, 256))) return rc;
+  constexpr size_t kCtrBytes = (size_t)kCtrSlots * kCtrStride * 8;
+  if ((rc = ensure(W.small, 256 + kCtrBytes))) return rc;
   if ((rc = ensure(W.hhead, N * 4 + 64))) return rc;
   if ((rc = ensure(W.hoff, (nvol + 1) * 8 + 64))) return rc;
 
@@ -2207,7 +2218,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   // small: [0] error [1] small dense cells [2] M [3] H [4] overflow [5] U
   //        [6] run-table fill [7] big dense cells
   uint32_t *small = static_cast<uint32_t *>(W.small.ptr);
-  unsigned long long *ctr = reinterpret_cast<unsigned long long *>(small + 16);  // 8 counters
+  unsigned long long *ctr = reinterpret_cast<unsigned long long *>(small + 64);  // slots
   const uint32_t NN = (uint32_t)N;
   const uint32_t stride = P->stride, offpos = P->off_pos, offvel = P->off_vel;
   const uint32_t gN = grid_for(N);
@@ -2235,7 +2246,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     return 0;
   };
 
-  FOF_HIP(hipMemsetAsync(small, 0, 256, st));
+  FOF_HIP(hipMemsetAsync(small, 0, 256 + kCtrBytes, st));
   Grid G;
   G.start = start;
   G.hkey = nullptr;
@@ -2495,9 +2506,11 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     if (label_out) FOF_HIP(hipMemcpyAsync(label_out, d_label, N * 4, hipMemcpyDeviceToHost, st));
   }
   if ((rc = te(14))) return rc;
-  unsigned long long hc[8];
-  FOF_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
+  unsigned long long hcs[kCtrSlots * kCtrStride], hc[4] = {0, 0, 0, 0};
+  FOF_HIP(hipMemcpyAsync(hcs, ctr, sizeof(hcs), hipMemcpyDeviceToHost, st));
   FOF_HIP(hipStreamSynchronize(st));
+  for (int q = 0; q < kCtrSlots; q++)
+    for (int k = 0; k < 4; k++) hc[k] += hcs[q * kCtrStride + k];
   if (stats) {
     stats->ndense = ndense + nbig;
     stats->pair_tests = hc[0];

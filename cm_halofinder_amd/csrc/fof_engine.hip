@@ -1218,9 +1218,10 @@ __device__ __forceinline__ bool runs_link(const float4 *__restrict__ pos4, uint3
   return false;
 }
 
-// Run pairs with more than kBigPair particle pairs are queued in LDS and
-// tested by 16-lane groups: 16 particle pairs per step, ballot exit.
-constexpr uint32_t kBigPair = 24;
+// Run pairs with more than BIGPAIR particle pairs (a template parameter of
+// k_link_dense) are queued in LDS and tested by 16-lane groups: 16 particle
+// pairs per step, ballot exit.  Smaller pairs are tested in place by the
+// thread that found them.
 
 // Home-cell sub-cell coordinates (0..11) packed 4 bits per axis.
 __device__ __forceinline__ uint16_t pack_home(int x, int y, int z) {
@@ -1258,7 +1259,7 @@ constexpr size_t dense_lds(uint32_t ra, uint32_t maxrunsb, bool sizes) {
 
 // WPE: waves per SIMD to fit (the register budget; LDS per workgroup is
 // sized at launch to the runs a cell of the batch can have).
-template <int NT, int WPE, int MAXRUNS, int MAXRUNSB, int BIGQ, int ROUND>
+template <int NT, int WPE, int MAXRUNS, int MAXRUNSB, int BIGQ, int ROUND, uint32_t BIGPAIR>
 __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restrict__ dense,
                                                     const uint32_t *__restrict__ ndense,
                                                     const float4 *__restrict__ pos4,
@@ -1397,7 +1398,7 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
       } else {
         ea = run_end[ha], eb = run_end[hb];
       }
-      if ((ea - ha) * (eb - hb) > kBigPair) {
+      if ((ea - ha) * (eb - hb) > BIGPAIR) {
         const uint32_t q = atomicAdd(&qn, 1u);
         if (q < (uint32_t)BIGQ) {
           bigq[q] = (sa << 16) | sb;
@@ -2410,7 +2411,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     if ((rc = tb(6))) return rc;
     constexpr int kRB = 448;
     const uint32_t ra = std::min(maxbins, (uint32_t)kMaxRuns);
-    k_link_dense<256, 8, kMaxRuns, kRB, 512, 512>
+    k_link_dense<256, 8, kMaxRuns, kRB, 512, 1024, 1>
         <<<std::min(nbig, 8192u), 256, dense_lds(ra, kRB, false), st>>>(
             dbig, small + 7, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol,
             ctr, small + 4, ra);
@@ -2421,7 +2422,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     if ((rc = tb(15))) return rc;
     constexpr int kRB = 64;
     const uint32_t ra = std::min(maxbins, kBigCell - 1);
-    k_link_dense<64, 6, (int)kBigCell - 1, kRB, 64, 512>
+    k_link_dense<64, 6, (int)kBigCell - 1, kRB, 64, 256, 8>
         <<<std::min(ndense, 65536u), 64, dense_lds(ra, kRB, true), st>>>(
             dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol,
             ctr, small + 4, ra);

@@ -1147,9 +1147,9 @@ __global__ __launch_bounds__(256, 8) void k_link_sparse(uint32_t N, const float4
 constexpr int kMaxRuns = 1728;
 constexpr uint32_t kMaxT = 512;
 // Default dense threshold: on clumpy 1024^3 boxes (b = 0.2, about one
-// particle per cell) the call time is flat within 1% from 24 to 32 and 5%
-// higher at 48.
-constexpr uint32_t kDefaultT = 32;
+// particle per cell) the link and sub-sort phases take 148-149 ms for T = 16,
+// 20 and 24, 153 ms at 32 and 155 ms at 40.
+constexpr uint32_t kDefaultT = 24;
 
 __device__ __forceinline__ uint32_t pack_rel(int a, int b, int c) {
   return ((uint32_t)(a & 255)) | ((uint32_t)(b & 255) << 8) | ((uint32_t)(c & 255) << 16);

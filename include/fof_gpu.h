@@ -107,7 +107,7 @@ typedef struct fof_opts_v1 {
   double band;              /* relative FP32 band around l^2 that is re-decided
                                in FP64; 0 selects 1e-5; must lie in [1e-6,0.1] */
   uint32_t dense_threshold; /* cells with at least this many particles are
-                               linked at sub-cell level; 0 = default (32),
+                               linked at sub-cell level; 0 = default (24),
                                values above 512 act as 512                   */
   uint32_t reserved;
 } fof_opts_v1;

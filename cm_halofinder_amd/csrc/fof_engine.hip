@@ -1737,13 +1737,16 @@ __global__ __launch_bounds__(256) void k_out_init(uint32_t N, const uint32_t *__
 }
 
 // Ring in ascending index order over the label-sorted list, and halo starts.
+// The ring entries leave as (index, successor) pairs in list order (pidx,
+// pval); scatter_u32 puts them into next_out.
 __global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
                                               const uint32_t *__restrict__ skey,
                                               const uint32_t *__restrict__ sval,
                                               const uint32_t *__restrict__ hsize,
                                               const VolDev *__restrict__ vols,
                                               const uint32_t *__restrict__ vbeg, int nvol,
-                                              uint32_t *__restrict__ next_out,
+                                              uint32_t *__restrict__ pidx,
+                                              uint32_t *__restrict__ pval,
                                               uint32_t *__restrict__ hflag) {
   const uint32_t M = *dM;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < M;
@@ -1751,9 +1754,10 @@ __global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
     const uint32_t lab = skey[k], i = sval[k];
     const int v = vol_of(vbeg, nvol, i);
     const uint32_t b = vbeg[v];
-    if (next_out) {
+    if (pidx) {
       const uint32_t nx = (k + 1 < M && skey[k + 1] == lab) ? sval[k + 1] : lab;
-      next_out[i] = nx - b + 1u;
+      pidx[k] = i;
+      pval[k] = nx - b + 1u;
     }
     const bool gstart = (k == 0) || (skey[k - 1] != lab);
     hflag[k] = (gstart && !((double)hsize[lab] < vols[v].nphalomin)) ? kFlag : 0u;
@@ -2460,8 +2464,15 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   uint64_t H = 0;
   if (M) {
     if ((rc = tb(11))) return rc;
-    k_ring<<<grid_for(M), 256, 0, st>>>(small + 2, sk0, sv0, hsize, dV, dVB, (int)nvol, d_next, sk1);
+    // ring pairs in the (now free) cell-sorted position buffer
+    uint32_t *p32 = reinterpret_cast<uint32_t *>(pos4);
+    uint32_t *pidx = d_next ? p32 : nullptr;
+    k_ring<<<grid_for(M), 256, 0, st>>>(small + 2, sk0, sv0, hsize, dV, dVB, (int)nvol, pidx,
+                                        p32 + N, sk1);
     FOF_HIP(hipGetLastError());
+    if (d_next &&
+        (rc = scatter_u32(p32, p32 + N, p32 + 2 * N, p32 + 3 * N, M, N, d_next, W.tmp.ptr, st)))
+      return rc;
     if ((rc = te(11))) return rc;
   }
   if (want_halos && M) {

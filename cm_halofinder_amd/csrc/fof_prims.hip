@@ -397,6 +397,42 @@ int radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint6
   return 0;
 }
 
+// ---- scatter through one partition pass -------------------------------------
+// One tile of kTile consecutive pairs per block: the blocks in flight then
+// cover one stretch of the partitioned list (a grid-stride loop would spread
+// them over many partitions).
+__global__ __launch_bounds__(kBlock) void k_put(const uint32_t *__restrict__ idx,
+                                                const uint32_t *__restrict__ val, uint64_t n,
+                                                uint32_t *__restrict__ out) {
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int r = 0; r < kItems; r++) {
+    const uint64_t i = base + (uint64_t)r * kBlock + threadIdx.x;
+    if (i < n) out[idx[i]] = val[i];
+  }
+}
+
+int scatter_u32(uint32_t *idx, uint32_t *val, uint32_t *idx2, uint32_t *val2, uint64_t n,
+                uint64_t nout, uint32_t *out, void *tmp, hipStream_t st) {
+  if (n == 0) return 0;
+  int bits = 0;
+  while (bits < 32 && (nout - 1) >> bits) bits++;
+  const int shift = bits > 8 ? bits - 8 : 0;
+  const uint64_t nb = (n + kTile - 1) / kTile;
+  uint32_t *hist = static_cast<uint32_t *>(tmp);
+  void *stmp = hist + 256 * nb + 64;
+  k_rs_hist<<<(uint32_t)nb, kBlock, 0, st>>>(idx, n, shift, hist, (uint32_t)nb);
+  FOF_HIP(hipGetLastError());
+  int rc = scan_u32(hist, hist, 256 * nb, stmp, st);
+  if (rc) return rc;
+  k_rs_scatter<<<(uint32_t)nb, kBlock, 0, st>>>(idx, val, idx2, val2, n, shift, hist,
+                                                (uint32_t)nb);
+  FOF_HIP(hipGetLastError());
+  k_put<<<(uint32_t)nb, kBlock, 0, st>>>(idx2, val2, n, out);
+  FOF_HIP(hipGetLastError());
+  return 0;
+}
+
 int ensure(Scratch &s, size_t bytes) {
   if (bytes <= s.bytes) return 0;
   if (s.ptr) FOF_HIP(hipFree(s.ptr));

@@ -1681,29 +1681,60 @@ __global__ __launch_bounds__(256) void k_flatten(uint32_t N, uint32_t *__restric
   if ((threadIdx.x & 63) == 0 && t) ctr_add(ctr, 2, t);
 }
 
-// label[orig] = group minimum index (global), with kFlag when the group is
-// kept for the ring sort (size >= 2, or size >= nphalomin).  The head's
-// entry of hsize receives the group size.
+// Members of groups of two or more (always kept for the ring sort): the
+// pair (orig, group minimum index | kFlag) for label[orig], appended to
+// (pidx, pval) -- gathered per tile of kLabTile particles in LDS and
+// appended with one atomic per tile; scatter_u32 puts them into label.  The
+// head's entry of hsize receives the group size.
+constexpr uint32_t kLabTile = 4096;
+
 __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__restrict__ par,
                                                const float4 *__restrict__ pos4,
                                                const uint32_t *__restrict__ minorig,
                                                const uint32_t *__restrict__ gsize,
-                                               const VolDev *__restrict__ vols,
-                                               const uint32_t *__restrict__ vbeg, int nvol,
-                                               uint32_t *__restrict__ label,
+                                               uint32_t *__restrict__ pidx,
+                                               uint32_t *__restrict__ pval,
+                                               uint32_t *__restrict__ npairs,
                                                uint32_t *__restrict__ hsize) {
-  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
-       x += gridDim.x * blockDim.x) {
-    const uint32_t r = par[x];
-    const uint32_t g = gsize[r];
-    if (r == x && g == 0) continue;  // a singleton: k_label_init wrote it
-    const uint32_t o = __float_as_uint(pos4[x].w);
-    const uint32_t orr = (r == x) ? o : __float_as_uint(pos4[r].w);
-    const uint32_t lab = min(minorig[r], orr), sz = g + 1u;
-    const VolDev &V = vols[vol_of(vbeg, nvol, x)];
-    const bool keep = (sz >= 2u) || !((double)sz < V.nphalomin);
-    label[o] = lab | (keep ? kFlag : 0u);
-    if (o == lab) hsize[lab] = sz;
+  __shared__ uint32_t lk[kLabTile], lv[kLabTile];
+  __shared__ uint32_t ln, lbase;
+  const uint32_t lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) ln = 0;
+  __syncthreads();
+  const uint64_t x0 = (uint64_t)blockIdx.x * kLabTile;
+  for (uint32_t q = 0; q < kLabTile; q += 256) {
+    const uint64_t xx = x0 + q + threadIdx.x;
+    bool mem = false;
+    uint32_t o = 0, lab = 0;
+    if (xx < N) {
+      const uint32_t x = (uint32_t)xx;
+      const uint32_t r = par[x];
+      const uint32_t g = gsize[r];
+      if (!(r == x && g == 0)) {  // singletons: k_label_init wrote them
+        mem = true;
+        o = __float_as_uint(pos4[x].w);
+        const uint32_t orr = (r == x) ? o : __float_as_uint(pos4[r].w);
+        lab = min(minorig[r], orr);
+        if (o == lab) hsize[lab] = g + 1u;
+      }
+    }
+    const unsigned long long b = __ballot(mem);
+    if (!b) continue;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&ln, (uint32_t)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (mem) {
+      const uint32_t j = base + (uint32_t)__popcll(b & ((1ull << lane) - 1));
+      lk[j] = o;
+      lv[j] = lab | kFlag;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) lbase = ln ? atomicAdd(npairs, ln) : 0u;
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < ln; j += 256) {
+    pidx[lbase + j] = lk[j];
+    pval[lbase + j] = lv[j];
   }
 }
 
@@ -2221,7 +2252,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   uint32_t *dense = static_cast<uint32_t *>(W.dense.ptr);
   uint32_t *dbig = dense + ndense_cap;  // cells of kBigCell particles or more
   // small: [0] error [1] small dense cells [2] M [3] H [4] overflow [5] U
-  //        [6] run-table fill [7] big dense cells
+  //        [6] run-table fill [7] big dense cells [8] group members (label pairs)
   uint32_t *small = static_cast<uint32_t *>(W.small.ptr);
   unsigned long long *ctr = reinterpret_cast<unsigned long long *>(small + 64);  // slots
   const uint32_t NN = (uint32_t)N;
@@ -2260,7 +2291,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   G.hmask = 0;
   G.sparse = sparse ? 1 : 0;
   G.volshift = volshift;
-  uint32_t hsmall[8];
+  uint32_t hsmall[16];
   uint32_t ndense = 0, nbig = 0;  // dense cells below / from kBigCell particles
   if (!sparse) {
     // 1-3: counting sort by cell: partitions A (coarse) and B (fine), then C
@@ -2444,8 +2475,15 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   if ((rc = tb(8))) return rc;
   k_label_init<<<gN, 256, 0, st>>>(NN, dV, dVB, (int)nvol, label, hsize);
   FOF_HIP(hipGetLastError());
-  k_label<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, dV, dVB, (int)nvol, label, hsize);
-  FOF_HIP(hipGetLastError());
+  {
+    const uint64_t nt = (N + kLabTile - 1) / kLabTile;
+    k_label<<<(uint32_t)nt, 256, 0, st>>>(NN, par, pos4, minorig, gsize, sk0, sv0, small + 8,
+                                          hsize);
+    FOF_HIP(hipGetLastError());
+    FOF_HIP(hipMemcpyAsync(hsmall, small, 16 * 4, hipMemcpyDeviceToHost, st));
+    FOF_HIP(hipStreamSynchronize(st));
+    if ((rc = scatter_u32(sk0, sv0, sk1, sv1, hsmall[8], N, label, W.tmp.ptr, st))) return rc;
+  }
   k_out_init<<<gN, 256, 0, st>>>(NN, label, dVB, (int)nvol, d_label, d_next);
   FOF_HIP(hipGetLastError());
   if ((rc = te(8))) return rc;

@@ -52,6 +52,41 @@ __device__ __forceinline__ int vol_of(const uint32_t *__restrict__ vbeg, int nvo
   return lo;
 }
 
+// Volume of element p when the wave's active lanes hold consecutive elements
+// (grid-stride loops over 64-aligned runs, tiles): the volumes of the wave's
+// first and last element are found with scalar loads, and the lanes only
+// search between them (in a batch nearly every wave lies in one volume).
+__device__ __forceinline__ int vol_of_run(const uint32_t *__restrict__ vbeg, int nvol,
+                                          uint32_t p, uint32_t n) {
+  if (nvol == 1) return 0;
+  const uint32_t a = __builtin_amdgcn_readfirstlane(p);
+  const uint32_t b = min(a + 63u, n - 1u);
+  int lo = vol_of(vbeg, nvol, a), hi = vol_of(vbeg, nvol, b);
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (vbeg[mid] <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The same when the active lanes' elements ascend with the lane index (a
+// sorted list): the first and the last active lane bound the wave's range.
+__device__ __forceinline__ int vol_of_sorted(const uint32_t *__restrict__ vbeg, int nvol,
+                                             uint32_t p) {
+  if (nvol == 1) return 0;
+  const uint32_t a = __builtin_amdgcn_readfirstlane(p);
+  const int last = 63 - __clzll(__ballot(1));
+  const uint32_t b = __builtin_amdgcn_readlane(p, last);
+  int lo = vol_of(vbeg, nvol, a), hi = vol_of(vbeg, nvol, b);
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (vbeg[mid] <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Reference cell coordinate along one axis (src_v2/FOF.c:79-81), clamped.
 __device__ __forceinline__ int cell1(double x, double rmin_buff, double lcell, int n,
                                      double *u) {
@@ -381,7 +416,7 @@ __global__ __launch_bounds__(256) void k_pa_count(const uint8_t *__restrict__ ba
   __syncthreads();
   const uint32_t p0 = tile * kPartTile, p1 = min(N, p0 + kPartTile);
   for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const uint32_t c = cell_of_record(base, stride, offpos, p, vols[vol_of(vbeg, nvol, p)], err);
+    const uint32_t c = cell_of_record(base, stride, offpos, p, vols[vol_of_run(vbeg, nvol, p, N)], err);
     key[p] = c;
     atomicAdd(&h[c >> (kFineBits + 8)], 1u);
   }
@@ -990,7 +1025,7 @@ __global__ __launch_bounds__(256, 8) void k_link_sparse(uint32_t N, const float4
   for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
        x += gridDim.x * blockDim.x) {
     const float4 px = pos4[x];
-    const VolDev &V = vols[vol_of(vbeg, nvol, x)];
+    const VolDev &V = vols[vol_of_run(vbeg, nvol, x, N)];
     const CellPos cp = cell_pos(V, px.x, px.y, px.z);
     const uint32_t c = G.sparse ? G.cell_of[x] : cell_id(V, cp.ix, cp.iy, cp.iz);
     const uint32_t s = G.start[c], e = G.start[c + 1];
@@ -1746,7 +1781,7 @@ __global__ __launch_bounds__(256) void k_label_init(uint32_t N, const VolDev *__
                                                     uint32_t *__restrict__ hsize) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += gridDim.x * blockDim.x) {
-    const bool keep = !(1.0 < vols[vol_of(vbeg, nvol, i)].nphalomin);
+    const bool keep = !(1.0 < vols[vol_of_run(vbeg, nvol, i, N)].nphalomin);
     label[i] = i | (keep ? kFlag : 0u);
     hsize[i] = 1u;
   }
@@ -1761,7 +1796,7 @@ __global__ __launch_bounds__(256) void k_out_init(uint32_t N, const uint32_t *__
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += gridDim.x * blockDim.x) {
     const uint32_t l = label[i];
-    const uint32_t b = vbeg[vol_of(vbeg, nvol, i)];
+    const uint32_t b = vbeg[vol_of_run(vbeg, nvol, i, N)];
     if (label_out) label_out[i] = (l & ~kFlag) - b;
     if (next_out && !(l & kFlag)) next_out[i] = i - b + 1u;
   }
@@ -1783,7 +1818,8 @@ __global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < M;
        k += gridDim.x * blockDim.x) {
     const uint32_t lab = skey[k], i = sval[k];
-    const int v = vol_of(vbeg, nvol, i);
+    // the list is sorted by label, so by volume; a member shares its head's volume
+    const int v = vol_of_sorted(vbeg, nvol, lab);
     const uint32_t b = vbeg[v];
     if (pidx) {
       const uint32_t nx = (k + 1 < M && skey[k + 1] == lab) ? sval[k + 1] : lab;

@@ -52,16 +52,24 @@ __device__ __forceinline__ int vol_of(const uint32_t *__restrict__ vbeg, int nvo
   return lo;
 }
 
-// Volume of element p when the wave's active lanes hold consecutive elements
-// (grid-stride loops over 64-aligned runs, tiles): the volumes of the wave's
-// first and last element are found with scalar loads, and the lanes only
-// search between them (in a batch nearly every wave lies in one volume).
-__device__ __forceinline__ int vol_of_run(const uint32_t *__restrict__ vbeg, int nvol,
-                                          uint32_t p, uint32_t n) {
-  if (nvol == 1) return 0;
-  const uint32_t a = __builtin_amdgcn_readfirstlane(p);
-  const uint32_t b = min(a + 63u, n - 1u);
-  int lo = vol_of(vbeg, nvol, a), hi = vol_of(vbeg, nvol, b);
+// Volume lookups for waves whose active lanes hold ascending elements (a
+// grid-stride loop over 64-aligned runs, a tile, a label-sorted list): the
+// volumes of the wave's first and last element are found with scalar loads,
+// starting from the previous iteration's volume (hint, wave-uniform), and the
+// lanes only search between them -- in a batch nearly every wave lies in one
+// volume, and a cold binary search is six dependent loads.
+__device__ __forceinline__ int vol_from(const uint32_t *__restrict__ vbeg, int nvol, uint32_t a,
+                                        int hint) {
+  if (vbeg[hint] <= a && a < vbeg[hint + 1]) return hint;
+  return vol_of(vbeg, nvol, a);
+}
+
+__device__ __forceinline__ int vol_between(const uint32_t *__restrict__ vbeg, int nvol,
+                                           uint32_t p, uint32_t a, uint32_t b, int &hint) {
+  const int lo0 = vol_from(vbeg, nvol, a, hint);
+  const int hi0 = (b < vbeg[lo0 + 1]) ? lo0 : vol_of(vbeg, nvol, b);
+  hint = hi0;
+  int lo = lo0, hi = hi0;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (vbeg[mid] <= p) lo = mid;
@@ -70,21 +78,21 @@ __device__ __forceinline__ int vol_of_run(const uint32_t *__restrict__ vbeg, int
   return lo;
 }
 
-// The same when the active lanes' elements ascend with the lane index (a
-// sorted list): the first and the last active lane bound the wave's range.
+// lanes hold consecutive elements p (n elements in all)
+__device__ __forceinline__ int vol_of_run(const uint32_t *__restrict__ vbeg, int nvol,
+                                          uint32_t p, uint32_t n, int &hint) {
+  if (nvol == 1) return 0;
+  const uint32_t a = __builtin_amdgcn_readfirstlane(p);
+  return vol_between(vbeg, nvol, p, a, min(a + 63u, n - 1u), hint);
+}
+
+// lanes hold ascending elements p
 __device__ __forceinline__ int vol_of_sorted(const uint32_t *__restrict__ vbeg, int nvol,
-                                             uint32_t p) {
+                                             uint32_t p, int &hint) {
   if (nvol == 1) return 0;
   const uint32_t a = __builtin_amdgcn_readfirstlane(p);
   const int last = 63 - __clzll(__ballot(1));
-  const uint32_t b = __builtin_amdgcn_readlane(p, last);
-  int lo = vol_of(vbeg, nvol, a), hi = vol_of(vbeg, nvol, b);
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (vbeg[mid] <= p) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
+  return vol_between(vbeg, nvol, p, a, __builtin_amdgcn_readlane(p, last), hint);
 }
 
 // Reference cell coordinate along one axis (src_v2/FOF.c:79-81), clamped.
@@ -415,8 +423,9 @@ __global__ __launch_bounds__(256) void k_pa_count(const uint8_t *__restrict__ ba
   for (uint32_t b = threadIdx.x; b < Bc; b += blockDim.x) h[b] = 0;
   __syncthreads();
   const uint32_t p0 = tile * kPartTile, p1 = min(N, p0 + kPartTile);
+  int hint = 0;
   for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const uint32_t c = cell_of_record(base, stride, offpos, p, vols[vol_of_run(vbeg, nvol, p, N)], err);
+    const uint32_t c = cell_of_record(base, stride, offpos, p, vols[vol_of_run(vbeg, nvol, p, N, hint)], err);
     key[p] = c;
     atomicAdd(&h[c >> (kFineBits + 8)], 1u);
   }
@@ -1022,10 +1031,11 @@ __global__ __launch_bounds__(256, 8) void k_link_sparse(uint32_t N, const float4
                                                      int nvol, uint32_t T,
                                                      unsigned long long *__restrict__ ctr) {
   uint32_t tests = 0, rechecks = 0;
+  int hint = 0;
   for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N;
        x += gridDim.x * blockDim.x) {
     const float4 px = pos4[x];
-    const VolDev &V = vols[vol_of_run(vbeg, nvol, x, N)];
+    const VolDev &V = vols[vol_of_run(vbeg, nvol, x, N, hint)];
     const CellPos cp = cell_pos(V, px.x, px.y, px.z);
     const uint32_t c = G.sparse ? G.cell_of[x] : cell_id(V, cp.ix, cp.iy, cp.iz);
     const uint32_t s = G.start[c], e = G.start[c + 1];
@@ -1779,9 +1789,10 @@ __global__ __launch_bounds__(256) void k_label_init(uint32_t N, const VolDev *__
                                                     const uint32_t *__restrict__ vbeg, int nvol,
                                                     uint32_t *__restrict__ label,
                                                     uint32_t *__restrict__ hsize) {
+  int hint = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += gridDim.x * blockDim.x) {
-    const bool keep = !(1.0 < vols[vol_of_run(vbeg, nvol, i, N)].nphalomin);
+    const bool keep = !(1.0 < vols[vol_of_run(vbeg, nvol, i, N, hint)].nphalomin);
     label[i] = i | (keep ? kFlag : 0u);
     hsize[i] = 1u;
   }
@@ -1793,10 +1804,11 @@ __global__ __launch_bounds__(256) void k_out_init(uint32_t N, const uint32_t *__
                                                   const uint32_t *__restrict__ vbeg, int nvol,
                                                   uint32_t *__restrict__ label_out,
                                                   uint32_t *__restrict__ next_out) {
+  int hint = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += gridDim.x * blockDim.x) {
     const uint32_t l = label[i];
-    const uint32_t b = vbeg[vol_of_run(vbeg, nvol, i, N)];
+    const uint32_t b = vbeg[vol_of_run(vbeg, nvol, i, N, hint)];
     if (label_out) label_out[i] = (l & ~kFlag) - b;
     if (next_out && !(l & kFlag)) next_out[i] = i - b + 1u;
   }
@@ -1815,11 +1827,12 @@ __global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
                                               uint32_t *__restrict__ pval,
                                               uint32_t *__restrict__ hflag) {
   const uint32_t M = *dM;
+  int hint = 0;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < M;
        k += gridDim.x * blockDim.x) {
     const uint32_t lab = skey[k], i = sval[k];
     // the list is sorted by label, so by volume; a member shares its head's volume
-    const int v = vol_of_sorted(vbeg, nvol, lab);
+    const int v = vol_of_sorted(vbeg, nvol, lab, hint);
     const uint32_t b = vbeg[v];
     if (pidx) {
       const uint32_t nx = (k + 1 < M && skey[k + 1] == lab) ? sval[k + 1] : lab;

@@ -148,10 +148,17 @@ __device__ __forceinline__ uint32_t cell_id(const VolDev &V, int ix, int iy, int
 // volume (cell id = cbeg + linear index).  Sparse: only occupied cells, in
 // (volume, linear index) order, found through an open-addressing hash table
 // keyed by the composite key (volume << volshift) | linear index.
+// A hash slot: the composite key (empty = ~0) and the cell index side by side,
+// so that a probe is one 16-byte load (the table is gigabytes: every probe
+// misses the caches).
+struct __align__(16) HEntry {
+  unsigned long long k;
+  unsigned long long v;
+};
+
 struct Grid {
   const uint32_t *start;           // cell start offsets (+1 entry)
-  const unsigned long long *hkey;  // sparse: hash keys (empty = ~0)
-  const uint32_t *hval;            // sparse: unique cell index
+  const struct HEntry *htab;       // sparse: hash table (key, cell index)
   const uint32_t *cell_of;         // sparse: cell of each sorted particle
   unsigned long long hmask;
   int sparse;
@@ -175,8 +182,9 @@ __device__ __forceinline__ unsigned long long comp_key(const Grid &G, const VolD
 __device__ __forceinline__ uint32_t hlookup(const Grid &G, unsigned long long k) {
   unsigned long long i = mix64d(k) & G.hmask;
   for (;;) {
-    const unsigned long long kk = G.hkey[i];
-    if (kk == k) return G.hval[i];
+    const HEntry h = G.htab[i];
+    const unsigned long long kk = h.k;
+    if (kk == k) return (uint32_t)h.v;
     if (kk == ~0ull) return kNone;
     i = (i + 1) & G.hmask;
   }
@@ -826,29 +834,50 @@ __global__ __launch_bounds__(256) void k_cell_flags(uint32_t N, const uint32_t *
 __global__ __launch_bounds__(256) void k_cells_finish(
     const uint32_t *__restrict__ dU, uint32_t N, uint32_t *__restrict__ cstart,
     const uint32_t *__restrict__ sklo, const uint32_t *__restrict__ skhi, Grid G,
-    unsigned long long *__restrict__ hkey, uint32_t *__restrict__ hval,
+    HEntry *__restrict__ htab,
     uint32_t *__restrict__ cell_of, const VolDev *__restrict__ vols,
     const uint32_t *__restrict__ vbeg, int nvol, uint32_t T, uint32_t *__restrict__ dense,
     uint32_t *__restrict__ ndense, uint32_t *__restrict__ dbig, uint32_t *__restrict__ nbig) {
   const uint32_t U = *dU;
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < U;
-       c += gridDim.x * blockDim.x) {
-    const uint32_t s = cstart[c], e = (c + 1 < U) ? cstart[c + 1] : N;
-    if (c + 1 == U) cstart[U] = N;
-    for (uint32_t x = s; x < e; x++) cell_of[x] = c;
-    const unsigned long long k = (unsigned long long)sklo[s] | ((unsigned long long)skhi[s] << 32);
-    unsigned long long i = mix64d(k) & G.hmask;
-    for (;;) {
-      if (atomicCAS(hkey + i, ~0ull, k) == ~0ull) {
-        hval[i] = c;
-        break;
+  const uint32_t lane = threadIdx.x & 63;
+  int hint = 0;
+  // wave-uniform trip count: the dense-cell lists are appended per wave
+  for (uint32_t cb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); cb < U;
+       cb += gridDim.x * blockDim.x) {
+    const uint32_t c = cb + lane;
+    bool sml = false, big = false;
+    if (c < U) {
+      const uint32_t s = cstart[c], e = (c + 1 < U) ? cstart[c + 1] : N;
+      if (c + 1 == U) cstart[U] = N;
+      for (uint32_t x = s; x < e; x++) cell_of[x] = c;
+      const unsigned long long k = (unsigned long long)sklo[s] | ((unsigned long long)skhi[s] << 32);
+      unsigned long long i = mix64d(k) & G.hmask;
+      for (;;) {
+        if (atomicCAS(&htab[i].k, ~0ull, k) == ~0ull) {
+          htab[i].v = c;
+          break;
+        }
+        i = (i + 1) & G.hmask;
       }
-      i = (i + 1) & G.hmask;
+      const VolDev &V = vols[vol_of_sorted(vbeg, nvol, s, hint)];  // s ascends with c
+      if (V.m[0] > 0 && e - s >= T) {
+        big = e - s >= kBigCell;
+        sml = !big;
+      }
     }
-    const VolDev &V = vols[vol_of(vbeg, nvol, s)];
-    if (V.m[0] > 0 && e - s >= T) {
-      if (e - s >= kBigCell) dbig[atomicAdd(nbig, 1u)] = c;
-      else dense[atomicAdd(ndense, 1u)] = c;
+    const unsigned long long bs = __ballot(sml), bb = __ballot(big);
+    const unsigned long long below = (1ull << lane) - 1;
+    if (bs) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(ndense, (uint32_t)__popcll(bs));
+      base = __shfl(base, 0, 64);
+      if (sml) dense[base + __popcll(bs & below)] = c;
+    }
+    if (bb) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(nbig, (uint32_t)__popcll(bb));
+      base = __shfl(base, 0, 64);
+      if (big) dbig[base + __popcll(bb & below)] = c;
     }
   }
 }
@@ -2334,8 +2363,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   FOF_HIP(hipMemsetAsync(small, 0, 256 + kCtrBytes, st));
   Grid G;
   G.start = start;
-  G.hkey = nullptr;
-  G.hval = nullptr;
+  G.htab = nullptr;
   G.cell_of = label;  // sparse: the label buffer is free until k_label
   G.hmask = 0;
   G.sparse = sparse ? 1 : 0;
@@ -2441,14 +2469,12 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     const uint32_t U = hsmall[5];
     uint64_t hsz = 1024;
     while (hsz < 2ull * U) hsz <<= 1;
-    if ((rc = ensure(W.hash, hsz * 12 + 256))) return rc;
-    unsigned long long *hk = static_cast<unsigned long long *>(W.hash.ptr);
-    uint32_t *hvv = reinterpret_cast<uint32_t *>(hk + hsz);
-    FOF_HIP(hipMemsetAsync(hk, 0xFF, hsz * 8, st));
-    G.hkey = hk;
-    G.hval = hvv;
+    if ((rc = ensure(W.hash, hsz * sizeof(HEntry) + 256))) return rc;
+    HEntry *htab = static_cast<HEntry *>(W.hash.ptr);
+    FOF_HIP(hipMemsetAsync(htab, 0xFF, hsz * sizeof(HEntry), st));
+    G.htab = htab;
     G.hmask = hsz - 1;
-    k_cells_finish<<<grid_for(U), 256, 0, st>>>(small + 5, NN, start, par, runend, G, hk, hvv, label,
+    k_cells_finish<<<grid_for(U), 256, 0, st>>>(small + 5, NN, start, par, runend, G, htab, label,
                                                 dV, dVB, (int)nvol, T, dense, small + 1, dbig,
                                                 small + 7);
     FOF_HIP(hipGetLastError());

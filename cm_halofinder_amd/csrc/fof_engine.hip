@@ -2000,45 +2000,57 @@ struct HostVol {
 
 // Smallest l^2 of the cells holding particles, per volume with a variable
 // l (as the bits of a non-negative double: their order is the doubles').
+// Each wave takes kVlRun consecutive particles and keeps a running minimum
+// per lane, flushed when the lane's volume changes and once at the end (an
+// atomic per wave and step on one address per volume serialises).
+constexpr uint32_t kVlRun = 2048;
+
 __global__ __launch_bounds__(256) void k_varlink_min(const uint8_t *__restrict__ base,
                                                      uint32_t stride, uint32_t offpos, uint32_t N,
                                                      const VolDev *__restrict__ vols,
                                                      const uint32_t *__restrict__ vbeg, int nvol,
                                                      unsigned long long *__restrict__ vmin) {
   const int lane = threadIdx.x & 63;
-  for (uint32_t b0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b0 < N;
-       b0 += gridDim.x * blockDim.x) {
-    const uint32_t p = b0 + lane;
-    int v = -1;
+  const uint64_t w0 = ((uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kVlRun;
+  int cv = -1, hint = 0;
+  unsigned long long cm = ~0ull;
+  for (uint32_t r = 0; r < kVlRun; r += 64) {
+    const uint64_t pp64 = w0 + r + lane;
+    if (pp64 >= N) break;
+    const uint32_t p = (uint32_t)pp64;
+    const int v = vol_of_run(vbeg, nvol, p, N, hint);
+    const VolDev &V = vols[v];
+    const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
+    const double x = pp[0], y = pp[1], z = pp[2];
+    const bool ok = V.vntab > 0 && x >= V.rmin_buff[0] && x <= V.rmax_buff[0] &&
+                    y >= V.rmin_buff[1] && y <= V.rmax_buff[1] && z >= V.rmin_buff[2] &&
+                    z <= V.rmax_buff[2];
     unsigned long long bits = ~0ull;
-    if (p < N) {
-      v = vol_of(vbeg, nvol, p);
-      const VolDev &V = vols[v];
-      const float *pp = reinterpret_cast<const float *>(base + (uint64_t)p * stride + offpos);
-      const double x = pp[0], y = pp[1], z = pp[2];
-      const bool ok = V.vntab > 0 && x >= V.rmin_buff[0] && x <= V.rmax_buff[0] &&
-                      y >= V.rmin_buff[1] && y <= V.rmax_buff[1] && z >= V.rmin_buff[2] &&
-                      z <= V.rmax_buff[2];
-      if (ok) {
-        double u;
-        const int ix = cell1(x, V.rmin_buff[0], V.lcell[0], V.nx, &u);
-        const int iy = cell1(y, V.rmin_buff[1], V.lcell[1], V.ny, &u);
-        const int iz = cell1(z, V.rmin_buff[2], V.lcell[2], V.nz, &u);
-        const double l2 = cell_linksq(V, ix, iy, iz);
-        memcpy(&bits, &l2, 8);
-      }
+    if (ok) {
+      double u;
+      const int ix = cell1(x, V.rmin_buff[0], V.lcell[0], V.nx, &u);
+      const int iy = cell1(y, V.rmin_buff[1], V.lcell[1], V.ny, &u);
+      const int iz = cell1(z, V.rmin_buff[2], V.lcell[2], V.nz, &u);
+      const double l2 = cell_linksq(V, ix, iy, iz);
+      memcpy(&bits, &l2, 8);
     }
-    const int v0 = __shfl(v, 0, 64);
-    if (__ballot(v != v0) == 0ull) {  // one volume: reduce over the wave first
+    if (v != cv) {
+      if (cv >= 0 && cm != ~0ull) atomicMin(vmin + cv, cm);
+      cv = v, cm = bits;
+    } else if (bits < cm) {
+      cm = bits;
+    }
+  }
+  const int v0 = __shfl(cv, 0, 64);
+  if (__ballot(cv != v0) == 0ull) {  // one volume: reduce over the wave first
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long t = __shfl_xor(bits, o, 64);
-        bits = t < bits ? t : bits;
-      }
-      if (lane == 0 && bits != ~0ull) atomicMin(vmin + v0, bits);
-    } else if (bits != ~0ull) {
-      atomicMin(vmin + v, bits);
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long t = __shfl_xor(cm, o, 64);
+      cm = t < cm ? t : cm;
     }
+    if (lane == 0 && v0 >= 0 && cm != ~0ull) atomicMin(vmin + v0, cm);
+  } else if (cv >= 0 && cm != ~0ull) {
+    atomicMin(vmin + cv, cm);
   }
 }
 
@@ -2294,7 +2306,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     // cell (only occupied cells are home cells of a pair), not of the table.
     unsigned long long *dmin = static_cast<unsigned long long *>(W.hoff.ptr);
     FOF_HIP(hipMemsetAsync(dmin, 0xFF, nvol * 8, st));
-    k_varlink_min<<<grid_for(N), 256, 0, st>>>(dbase, P->stride, P->off_pos, (uint32_t)N,
+    k_varlink_min<<<(uint32_t)((N + 4 * kVlRun - 1) / (4 * kVlRun)), 256, 0, st>>>(dbase, P->stride, P->off_pos, (uint32_t)N,
                                                static_cast<const VolDev *>(W.vols.ptr),
                                                static_cast<const uint32_t *>(W.vbeg.ptr),
                                                (int)nvol, dmin);

@@ -1905,43 +1905,54 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
                                                    fof_halo_v1 *__restrict__ out,
                                                    uint32_t *__restrict__ hhead,
                                                    unsigned long long *__restrict__ ctr) {
-  __shared__ float stage[4][6][64];
+  // two stages per wave: the records of the next 64 members are loaded while
+  // lanes 0-5 add up the current ones
+  __shared__ float stage[4][2][6][64];
   const uint32_t H = *dH;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t nw = gridDim.x * (blockDim.x >> 6);
   uint32_t owned_cnt = 0;
+  int hint = 0;
   for (uint32_t h = blockIdx.x * (blockDim.x >> 6) + w; h < H; h += nw) {
     const uint32_t k0 = hk[h];
     const uint32_t lab = skey[k0];
     const uint32_t n = hsize[lab];
-    const int v = vol_of(vbeg, nvol, lab);
+    hint = vol_from(vbeg, nvol, lab, hint);  // halos ascend by head, per wave too
+    const int v = hint;
     const VolDev &V = vols[v];
     // lanes 0-5 each accumulate one component in ring order (the host's
     // order); the boundary test of src_v2/FOF.c:403-422 uses the staged
     // positions of every member and is kept for owned halos only
     double acc = 0;
     uint32_t warn = 0;
-    for (uint32_t k = 0; k < n; k += 64) {
-      const uint32_t m = min(64u, n - k);
-      if ((uint32_t)lane < m) {
+    float r[6];
+    auto fetch = [&](uint32_t k) {
+      if (lane < min(64u, n - k)) {
         const uint8_t *rec = base + (uint64_t)sval[k0 + k + lane] * stride;
         const float *pp = reinterpret_cast<const float *>(rec + offpos);
         const float *vv = reinterpret_cast<const float *>(rec + offvel);
-        const float px = pp[0], py = pp[1], pz = pp[2];
-        stage[w][0][lane] = px, stage[w][1][lane] = py, stage[w][2][lane] = pz;
-        stage[w][3][lane] = vv[0], stage[w][4][lane] = vv[1], stage[w][5][lane] = vv[2];
-        const double q[3] = {px, py, pz};
+        r[0] = pp[0], r[1] = pp[1], r[2] = pp[2], r[3] = vv[0], r[4] = vv[1], r[5] = vv[2];
+      }
+    };
+    fetch(0);
+    int b = 0;
+    for (uint32_t k = 0; k < n; k += 64, b ^= 1) {
+      const uint32_t m = min(64u, n - k);
+      if ((uint32_t)lane < m) {
 #pragma unroll
-        for (int d = 0; d < 3; d++)
-          if ((q[d] - V.rmin_buff[d] < V.linklength) || (V.rmax_buff[d] - q[d] < V.linklength))
+        for (int c = 0; c < 6; c++) stage[w][b][c][lane] = r[c];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const double q = r[d];
+          if ((q - V.rmin_buff[d] < V.linklength) || (V.rmax_buff[d] - q < V.linklength))
             warn |= 1u << d;
+        }
       }
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (k + 64 < n) fetch(k + 64);
       if (lane < 6)
-        for (uint32_t q = 0; q < m; q++) acc += (double)stage[w][lane][q];
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (uint32_t q = 0; q < m; q++) acc += (double)stage[w][b][lane][q];
     }
     const double s0 = __shfl(acc, 0, 64), s1 = __shfl(acc, 1, 64), s2 = __shfl(acc, 2, 64);
     const double s3 = __shfl(acc, 3, 64), s4 = __shfl(acc, 4, 64), s5 = __shfl(acc, 5, 64);

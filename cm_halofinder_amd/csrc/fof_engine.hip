@@ -1412,30 +1412,36 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
     // (k_link_sparse leaves their pairs with dense cells to this kernel);
     // their items (runs of dense neighbours, particles of sparse ones) and
     // the window of sub-cells that can reach A
-    if (threadIdx.x < 26) {
+    if (threadIdx.x < 64) {  // wave 0: one neighbour a lane, then a scan of the counts
       const int k = threadIdx.x;
-      int dd[3];
-      nb_offset(k, dd);
-      const int bx = ix + dd[0], by = iy + dd[1], bz = iz + dd[2];
-      uint32_t sB = 0, eB = 0;
-      const bool want = k < 13 || (!G.sparse && V.vntab == 0);
-      if (want && !(bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz < 0 || bz >= V.nz)) {
-        const uint32_t cb = cell_index(G, V, bx, by, bz);
-        if (cb != kNone) sB = G.start[cb], eB = G.start[cb + 1];
+      uint32_t items = 0;
+      if (k < 26) {
+        int dd[3];
+        nb_offset(k, dd);
+        const int bx = ix + dd[0], by = iy + dd[1], bz = iz + dd[2];
+        uint32_t sB = 0, eB = 0;
+        const bool want = k < 13 || (!G.sparse && V.vntab == 0);
+        if (want && !(bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz < 0 || bz >= V.nz)) {
+          const uint32_t cb = cell_index(G, V, bx, by, bz);
+          if (cb != kNone) sB = G.start[cb], eB = G.start[cb + 1];
+        }
+        if (k >= 13 && eB - sB >= T) eB = sB;  // a dense backward cell links this pair itself
+        const bool dn = (eB - sB >= T);
+        nbS[k] = sB;
+        nbDense[k] = dn ? 1 : 0;
+        items = dn ? rtoff[sB] : eB - sB;  // runs of a dense cell, else particles
       }
-      if (k >= 13 && eB - sB >= T) eB = sB;  // a dense backward cell links this pair itself
-      const bool dn = (eB - sB >= T);
-      nbS[k] = sB;
-      nbDense[k] = dn ? 1 : 0;
-      nbOff[k + 1] = dn ? rtoff[sB] : eB - sB;  // runs of a dense cell, else particles
+#pragma unroll
+      for (int o = 1; o < 32; o <<= 1) {
+        const uint32_t t = __shfl_up(items, o, 64);
+        if (k >= o) items += t;
+      }
+      if (k < 26) nbOff[k + 1] = items;
+      if (k == 0) nbOff[0] = 0;
     }
     if (threadIdx.x == 0) qn = 0;
     if (threadIdx.x < kOccW) occ[threadIdx.x] = 0;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      nbOff[0] = 0;
-      for (int k = 0; k < 26; k++) nbOff[k + 1] += nbOff[k];
-    }
     const int wy = 2 * reach[1] + 1;
     // the home cell's runs (one per occupied sub-cell, in sub-cell order)
     // from its run table; runs <= min(particles, sub-cells) <= ra
@@ -1619,8 +1625,12 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
       if (threadIdx.x == 0) cnt = 0;
       __syncthreads();
       for (uint32_t g = c0 + threadIdx.x; g < c1; g += blockDim.x) {
-        int q = 0;
-        while (nbOff[q + 1] <= g) q++;
+        int q = 0, qh = 25;  // the last neighbour whose items start at or before g
+        while (q < qh) {
+          const int mid = (q + qh + 1) >> 1;
+          if (nbOff[mid] <= g) q = mid;
+          else qh = mid - 1;
+        }
         const uint32_t item = g - nbOff[q];
         const uint32_t j = nbDense[q] ? rtab[nbS[q] + item] : nbS[q] + item;
         int fx, fy, fz;

@@ -6,12 +6,14 @@
 // :378-430, :797-847) derive from the ring.
 //
 // Pipeline (one launch per phase for all volumes of a batch):
-//   1 k_part_count   reference cell of every particle (src_v2/FOF.c:79-83),
-//                    per-tile histograms of cell buckets
+//   1 k_pa_count     reference cell of every particle (src_v2/FOF.c:79-83),
+//                    per-tile histograms of coarse cell buckets
 //   2 scan           (bucket, tile) offsets
-//   3 k_part_scatter, k_bucket_sort   partition by bucket, then a counting
-//                    sort per bucket -> cell starts, dense-cell lists and
-//                    float4 {x,y,z,index} in cell order
+//   3 k_pa_scatter, k_pb_scatter, k_pc_sort   two LDS-staged partitions
+//                    (coarse, then 256 fine buckets of 2^14 cells), then a
+//                    counting sort per fine bucket -> cell starts, dense-cell
+//                    lists and float4 {x,y,z,index} in cell order
+//                    (sparse addressing: composite keys, radix sort, hash)
 //   4 k_subsort      cells with >= T particles: counting sort by sub-cell;
 //                    sub-cells have side <= l/sqrt(3), so each is a clique and
 //                    its particles start in one union-find set ("runs")
@@ -20,8 +22,10 @@
 //   6 k_link_dense   dense cells, one workgroup (256 particles or more) or
 //                    one wave each: run pairs with early exit
 //   7 union-find finish: flatten, group minimum index, sizes
-//   8 labels, compaction of kept groups, stable radix sort by label
-//   9 ring in ascending index order, halo list, FP64 halo sums
+//   8 labels (members' pairs through scatter_u32), compaction of kept
+//     groups, stable radix sort by label
+//   9 ring in ascending index order (through scatter_u32), halo list,
+//     FP64 halo sums
 //
 // Pair rule (src_v2/FOF.c:351-357): dx = fl32(xi - xj) widened to double,
 // distsq = (dx^2 + dy^2) + dz^2 in FP64, linked iff !(distsq > linksq).

@@ -310,6 +310,47 @@ def test_many_volumes(ctx):
         assert np.array_equal(hv["head"], oh["head"]) and np.array_equal(hv["sum_vel"], oh["sum_vel"])
 
 
+@pytest.mark.parametrize("flags", [0, L.OPT_SPARSE])
+def test_tiny_volumes(ctx, flags):
+    # 70 volumes of 1-90 particles: one wave of 64 consecutive particles
+    # spans several volumes, so every per-wave volume lookup searches
+    rng = np.random.default_rng(17)
+    sizes = rng.integers(1, 91, 70)
+    parts, vols, off = [], [], 0
+    for k, m in enumerate(sizes):
+        box = 3.0 + (k % 5)
+        pos = rng.random((m, 3)) * box
+        pos[: m // 2] = box / 2 + rng.normal(0, 0.15, (m // 2, 3))  # a clump in half of it
+        pos = np.clip(pos, 0, box * 0.999)
+        q = np.zeros(m, L.PART_DTYPE)
+        q["Pos"] = pos.astype(np.float32)
+        q["Vel"] = rng.normal(0, 1, (m, 3)).astype(np.float32)
+        parts.append(q)
+        v = L.Volume()
+        v.offset, v.n = off, m
+        v.domain = L.domain([0] * 3, [box] * 3, [0] * 3, [box] * 3, [0.5] * 3, 0.25 + 0.01 * (k % 3), 2)
+        vols.append(v)
+        off += m
+    P = np.concatenate(parts)
+    n = len(P)
+    nxt = np.empty(n, np.uint32)
+    lab = np.empty(n, np.uint32)
+    halos = np.empty(n, L.HALO_DTYPE)
+    hoff = np.zeros(len(vols) + 1, np.uint64)
+    L.check(ctx.link_batch(L.particles(P), vols, L.opts(flags, 0.0, 4), nxt, lab, halos, n, hoff))
+    for v, vol in enumerate(vols):
+        a, b = int(vol.offset), int(vol.offset + vol.n)
+        od = odomain(vol)
+        olab, _ = O.link_uf(P["Pos"][a:b], od)
+        assert np.array_equal(lab[a:b], olab), v
+        oring = O.ring_from_labels(olab)
+        assert np.array_equal(nxt[a:b].astype(np.int64) - 1, oring.astype(np.int64)), v
+        oh = O.retrieve(oring, P["Pos"][a:b], P["Vel"][a:b], od)
+        hv = halos[int(hoff[v]):int(hoff[v + 1])]
+        assert len(hv) == len(oh), v
+        assert np.array_equal(hv["head"], oh["head"]) and np.array_equal(hv["sum_pos"], oh["sum_pos"])
+
+
 @pytest.mark.parametrize("scale,b", [(32, 0.2), (20, 0.6), (24, 0.1)])
 def test_sparse_cells(ctx, scale, b):
     bx = W.plain(scale, seed=15, b=b)

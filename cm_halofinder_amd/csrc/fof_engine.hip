@@ -1840,6 +1840,13 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
   }
 }
 
+// Streaming kernels whose elements' volumes are looked up per wave take one
+// contiguous tile of kTileN elements per block, so that a wave's next
+// elements usually lie in the volume of its last ones (the hint of
+// vol_of_run); a grid-stride loop would jump across volumes every step.
+constexpr uint32_t kTileN = 4096;
+static inline uint32_t tiles_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + kTileN - 1) / kTileN); }
+
 // Every particle as a singleton (label = own index, group size 1, kept only
 // when Nphalomin <= 1); k_label overwrites the members of larger groups.
 __global__ __launch_bounds__(256) void k_label_init(uint32_t N, const VolDev *__restrict__ vols,
@@ -1847,8 +1854,8 @@ __global__ __launch_bounds__(256) void k_label_init(uint32_t N, const VolDev *__
                                                     uint32_t *__restrict__ label,
                                                     uint32_t *__restrict__ hsize) {
   int hint = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
-       i += gridDim.x * blockDim.x) {
+  const uint64_t i0 = (uint64_t)blockIdx.x * kTileN, i1 = min((uint64_t)N, i0 + kTileN);
+  for (uint32_t i = (uint32_t)i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const bool keep = !(1.0 < vols[vol_of_run(vbeg, nvol, i, N, hint)].nphalomin);
     label[i] = i | (keep ? kFlag : 0u);
     hsize[i] = 1u;
@@ -1862,8 +1869,8 @@ __global__ __launch_bounds__(256) void k_out_init(uint32_t N, const uint32_t *__
                                                   uint32_t *__restrict__ label_out,
                                                   uint32_t *__restrict__ next_out) {
   int hint = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
-       i += gridDim.x * blockDim.x) {
+  const uint64_t i0 = (uint64_t)blockIdx.x * kTileN, i1 = min((uint64_t)N, i0 + kTileN);
+  for (uint32_t i = (uint32_t)i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const uint32_t l = label[i];
     const uint32_t b = vbeg[vol_of_run(vbeg, nvol, i, N, hint)];
     if (label_out) label_out[i] = (l & ~kFlag) - b;
@@ -1885,8 +1892,8 @@ __global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
                                               uint32_t *__restrict__ hflag) {
   const uint32_t M = *dM;
   int hint = 0;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < M;
-       k += gridDim.x * blockDim.x) {
+  const uint64_t k0 = (uint64_t)blockIdx.x * kTileN, k1 = min((uint64_t)M, k0 + kTileN);
+  for (uint32_t k = (uint32_t)k0 + threadIdx.x; k < k1; k += blockDim.x) {
     const uint32_t lab = skey[k], i = sval[k];
     // the list is sorted by label, so by volume; a member shares its head's volume
     const int v = vol_of_sorted(vbeg, nvol, lab, hint);
@@ -2605,7 +2612,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   FOF_HIP(hipGetLastError());
   if ((rc = te(7))) return rc;
   if ((rc = tb(8))) return rc;
-  k_label_init<<<gN, 256, 0, st>>>(NN, dV, dVB, (int)nvol, label, hsize);
+  k_label_init<<<tiles_for(N), 256, 0, st>>>(NN, dV, dVB, (int)nvol, label, hsize);
   FOF_HIP(hipGetLastError());
   {
     const uint64_t nt = (N + kLabTile - 1) / kLabTile;
@@ -2616,7 +2623,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     FOF_HIP(hipStreamSynchronize(st));
     if ((rc = scatter_u32(sk0, sv0, sk1, sv1, hsmall[8], N, label, W.tmp.ptr, st))) return rc;
   }
-  k_out_init<<<gN, 256, 0, st>>>(NN, label, dVB, (int)nvol, d_label, d_next);
+  k_out_init<<<tiles_for(N), 256, 0, st>>>(NN, label, dVB, (int)nvol, d_label, d_next);
   FOF_HIP(hipGetLastError());
   if ((rc = te(8))) return rc;
   // 8: kept groups sorted by label (stable: ascending index inside a group)
@@ -2637,7 +2644,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     // ring pairs in the (now free) cell-sorted position buffer
     uint32_t *p32 = reinterpret_cast<uint32_t *>(pos4);
     uint32_t *pidx = d_next ? p32 : nullptr;
-    k_ring<<<grid_for(M), 256, 0, st>>>(small + 2, sk0, sv0, hsize, dV, dVB, (int)nvol, pidx,
+    k_ring<<<tiles_for(M), 256, 0, st>>>(small + 2, sk0, sv0, hsize, dV, dVB, (int)nvol, pidx,
                                         p32 + N, sk1);
     FOF_HIP(hipGetLastError());
     if (d_next &&

@@ -11,10 +11,14 @@ device-resident outputs (ring, labels, halo table):
              call (``fof_link_gpu_batch``)
 
 value = 1024^3 owned particles / seconds per batched step.  The plain path,
-the speedup, per-phase device times, a kernel roofline and a CPU baseline
-(the reference's serial algorithm restated in C, one core, on sub-boxes of
-the same realization, each also checked for bit-exact groups against the GPU)
-are reported on the same JSON line.
+the speedup, per-phase device times, the link kernels' roofline (performed
+pair tests) and the HBM rates of the sort and union-find passes, a parity
+sample of the timed 1024^3 output against the oracle with the halo-property
+error, the PCIe copy rates a host-buffer call would add, a CPU baseline (the
+reference's serial algorithm restated in C, one process per host core, on
+sub-boxes of the same realization, each also checked for bit-exact groups
+against the GPU) and the 4096-volume batch with its per-volume latencies are
+reported on the same JSON line.
 """
 from __future__ import annotations
 
@@ -146,52 +150,105 @@ def candidate_pairs(recs_u8, vols, T):
     return tot, nd
 
 
-def profiled_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary of this workload (FETCH_SIZE and WRITE_SIZE passes, scaled by a
-    4 GiB copy), or None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_hbm.txt")
+def hbm_profile(workload):
+    """Per-kernel HBM bytes per launch from the committed rocprofv3 PMC
+    summary of a workload (FETCH_SIZE and WRITE_SIZE passes scaled by a
+    4 GiB copy): {kernel: bytes}, or {} when there is none."""
+    path = os.path.join(ROOT, "profiles", f"hbm_{workload}.txt")
+    out = {}
     if not os.path.exists(path):
-        return None, None
+        return out, None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
-            return float(line.split(":")[1]), os.path.relpath(path, ROOT)
-    return None, None
+        if line.startswith("traffic "):
+            _, k, v = line.split()
+            out[k] = float(v)
+    return out, os.path.relpath(path, ROOT)
 
 
-def roofline_link(stats, L, recs_u8, vols, workload=None):
-    """Roofline of the dominant link kernel of a timed call: algorithmic FP32
-    flops (8 per candidate pair of the reference sweep) over the kernel's
-    device time from the library's HIP events on its stream."""
+# Per-particle algorithmic HBM bytes of the passes (the unit definitions of
+# the measurement plan): sort 28 + 4 cells/N, union-find 12, reductions 28
+# per halo member.
+B_UF, B_RED = 12.0, 28.0
+
+# Kernels of each pass, as rocprofv3 names them (prefix match).
+PASS_KERNELS = {
+    "sort": ["k_pa_count", "k_pa_starts", "k_pb_tiles", "k_pa_scatter", "k_pb_count", "k_pb_scatter",
+             "k_pb_starts", "k_pc_sort", "k_scan_reduce", "k_scan_single", "k_scan_down"],
+    "union_find": ["k_init_uf", "k_flatten", "k_label_init", "k_label", "k_put", "k_out_init"],
+    "halo_sums": ["k_halo_sums"],
+}
+
+
+def roofline(stats, L, recs_u8, vols, N, cells, workload=None):
+    """Rooflines of a timed call.
+
+    Linking: the dominant link kernel's performed particle-pair tests x 8
+    FP32 flop over its device time (HIP events on the library's stream),
+    against the FP32 vector peak; the pass as pair tests/s against
+    157.3e12 / 8.  The reference sweep's candidate pairs (src_v2/FOF.c:94-161)
+    are reported beside it as work the sub-cell cliques, gap pruning and
+    early exits avoided -- not as work done.
+    HBM passes: algorithmic bytes over the phases' device time, against 8 TB/s,
+    and the profiled traffic of their kernels when a summary is committed."""
     ph = dict(zip(L.PHASES, stats.phase_ms))
-    pairs, nd = candidate_pairs(recs_u8, vols, stats.dense_threshold)
-    cands = [("k_link_dense", "link_dense", "big"), ("k_link_dense_wave", "link_dense_wave", "wave"),
-             ("k_link_sparse", "link_sparse", "sparse")]
-    kern, phase, key = max(cands, key=lambda c: ph[c[1]])
-    ms = ph[phase]
-    tf = pairs[key] * 8 / (ms * 1e-3) / 1e12
-    traffic, src = profiled_traffic(kern, workload) if workload else (None, None)
-    return {
+    tests = dict(zip(L.LINK_KERNELS, [int(x) for x in stats.pair_tests_kernel]))
+    phase_of = {"k_link_sparse": "link_sparse", "k_link_dense": "link_dense",
+                "k_link_dense_wave": "link_dense_wave"}
+    kern = max(phase_of, key=lambda k: ph[phase_of[k]])
+    ms = ph[phase_of[kern]]
+    tf = tests[kern] * 8 / (ms * 1e-3) / 1e12
+    cand, nd = candidate_pairs(recs_u8, vols, stats.dense_threshold)
+    cand_by = {"k_link_dense": cand["big"], "k_link_dense_wave": cand["wave"],
+               "k_link_sparse": cand["sparse"]}
+    prof, src = hbm_profile(workload) if workload else ({}, None)
+    link_ms = ph["link_sparse"] + ph["link_dense"] + ph["link_dense_wave"]
+    total_tests = int(stats.pair_tests)
+    out = {
         "kernel": kern,
-        "bound": "mfma",
-        "compute": "FP32 vector ALU (no matrix work on this path); peak = MI355X FP32 157.3 TFLOP/s",
+        "bound": "valu",
+        "compute": "FP32 vector ALU, 8 flop per performed particle-pair test (3 sub, 1 mul, "
+                   "2 fma); peak = MI355X FP32 157.3 TFLOP/s; no matrix work on this path. The "
+                   "kernel is latency-bound (dependent union-find and run-table loads)",
         "achieved": tf,
         "peak": 157.3,
         "unit": "TFLOP/s",
         "frac": tf / 157.3,
-        "traffic": traffic,
+        "traffic": prof.get(kern),
         "traffic_source": src,
         "kernel_ms": ms,
-        "algorithmic_pairs": pairs[key],
-        "flop_per_pair": 8,
-        "pairs_by_kernel": {"k_link_dense": pairs["big"], "k_link_dense_wave": pairs["wave"],
-                            "k_link_sparse": pairs["sparse"]},
-        "kernel_ms_by_kernel": {"k_link_dense": ph["link_dense"],
-                                "k_link_dense_wave": ph["link_dense_wave"],
-                                "k_link_sparse": ph["link_sparse"]},
+        "pair_tests": tests[kern],
+        "candidates_pruned": cand_by[kern] - tests[kern],
+        "link_pass": {
+            "pair_tests": total_tests,
+            "pair_tests_by_kernel": tests,
+            "pairs_joined_without_test": int(stats.pairs_joined),
+            "ms": link_ms,
+            "pair_tests_per_s": total_tests / (link_ms * 1e-3),
+            "peak_pair_tests_per_s": 157.3e12 / 8,
+            "frac": total_tests / (link_ms * 1e-3) / (157.3e12 / 8),
+            "reference_candidate_pairs": int(sum(cand.values())),
+            "kernel_ms": {k: ph[v] for k, v in phase_of.items()},
+        },
         "dense_cells_check": {"counted": nd, "engine": int(stats.ndense)},
-        "pair_tests_performed": int(stats.pair_tests),
     }
+    # HBM passes
+    b_sort = 28.0 + 4.0 * cells / N
+    halo_members = None
+    passes = {
+        "sort": (b_sort * N, ph["keys"] + ph["scan"] + ph["scatter"], f"{b_sort:.2f} B x N"),
+        "union_find": (B_UF * N, ph["uf_init"] + ph["flatten"] + ph["label"], "12 B x N"),
+    }
+    hp = {}
+    for name, (byt, pms, unit) in passes.items():
+        gbs = byt / (pms * 1e-3) / 1e9
+        traf = sum(v for k, v in prof.items() if any(k.startswith(q) for q in PASS_KERNELS[name])) \
+            if prof else None
+        hp[name] = {"algorithmic_bytes": byt, "unit": unit, "ms": pms, "achieved_GBs": gbs,
+                    "peak_GBs": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
+                    "traffic_bytes": traf,
+                    "traffic_over_algorithmic": (traf / byt) if traf else None}
+    out["hbm_passes"] = hp
+    return out
 
 
 def run_steps(fn, warmup, steps):
@@ -207,61 +264,192 @@ def run_steps(fn, warmup, steps):
     return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(recs_u8, bx, budget_s, ctx, L, O):
-    """Reference algorithm (oracle.link_ring: src_v2/FOF.c:44-161 + Befriend,
-    serial) on interior sub-boxes of the same realization, one core.  Each
-    sub-box is also linked on the GPU and its labels compared bit for bit."""
+# ---- CPU baseline: the reference's serial algorithm on the host's cores ----
+def _cpu_worker(task):
+    """One sub-box through the oracle's ring linker (src_v2/FOF.c:44-161 +
+    Befriend, serial); runs in a worker process started before the GPU is
+    touched."""
+    import oracle as O
+
+    pos, dom = task
+    od = O.make_domain(*dom)
+    t0 = time.perf_counter()
+    nxt, _ = O.link_ring(pos, od)
+    dt = time.perf_counter() - t0
+    return O.labels_from_ring(nxt), dt
+
+
+def cpu_pool():
+    """Worker processes for the CPU baseline: one per host core (at most 16,
+    this machine's share), spawned before any GPU call."""
+    import multiprocessing as mp
+
+    n = max(1, min(16, os.cpu_count() or 1))
+    return mp.get_context("spawn").Pool(n), n
+
+
+def subbox(recs_u8, xyz, lo, hi):
+    """Records (and their indices in recs) with positions in [lo, hi]^3."""
+    import torch
+
+    lo_t = torch.tensor(lo - 1.0, device=xyz.device, dtype=torch.float32)
+    hi_t = torch.tensor(hi + 1.0, device=xyz.device, dtype=torch.float32)
+    m = ((xyz >= lo_t) & (xyz <= hi_t)).all(1)
+    idx = torch.nonzero(m).flatten()
+    del m
+    from cm_halofinder_amd import _lib as L
+
+    sub = recs_u8.view(-1, 32)[idx].cpu().numpy().reshape(-1).view(L.PART_DTYPE).copy()
+    idx = idx.cpu().numpy()
+    pp = sub["Pos"].astype(np.float64)
+    keep = ((pp >= lo) & (pp <= hi)).all(1)
+    return sub[keep].copy(), idx[keep]
+
+
+def cpu_baseline(recs_u8, bx, pool, ncores, ctx, L, budget_s=12.0):
+    """The reference's serial algorithm (oracle.link_ring) on interior
+    sub-boxes of the same realization, one process per core, scheduled until
+    about budget_s of wall time; every sub-box is also linked on the GPU and
+    its labels compared bit for bit."""
     import torch
 
     S, shell = 15.625, bx.shell
     rng = np.random.default_rng(2024)
     xyz = recs_u8.view(torch.float32).view(-1, 8)[:, 2:5]
-    done_owned, done_t, nbox, exact = 0, 0.0, 0, True
-    uf_t = 0.0
-    while done_t < budget_s and nbox < 64:
-        c = rng.uniform(S / 2 + shell + 1, bx.box - S / 2 - shell - 1, 3)
-        lo, hi = c - S / 2 - shell, c + S / 2 + shell
-        # coarse float32 cut on the device, exact double cut on the host
-        lo_t = torch.tensor(lo - 1.0, device=xyz.device, dtype=torch.float32)
-        hi_t = torch.tensor(hi + 1.0, device=xyz.device, dtype=torch.float32)
-        m = ((xyz >= lo_t) & (xyz <= hi_t)).all(1)
-        idx = torch.nonzero(m).flatten()
-        sub = recs_u8.view(-1, 32)[idx].cpu().numpy().reshape(-1).view(L.PART_DTYPE).copy()
-        del m
-        pp = sub["Pos"].astype(np.float64)
-        sub = sub[((pp >= lo) & (pp <= hi)).all(1)].copy()
-        rmin, rmax = c - S / 2, c + S / 2
-        lc = [bx.box / bx.pside] * 3
-        od = O.make_domain(rmin, rmax, lo, hi, lc, bx.linklength, bx.nphalomin)
-        t0 = time.perf_counter()
-        nxt, _ = O.link_ring(sub["Pos"], od)
-        dt = time.perf_counter() - t0
-        t1 = time.perf_counter()
-        ulab, _ = O.link_uf(sub["Pos"], od)
-        uf_t += time.perf_counter() - t1
-        p = sub["Pos"].astype(np.float64)
-        owned = int((((p >= rmin) & (p < rmax)).all(1)).sum())
-        done_owned += owned
-        done_t += dt
-        nbox += 1
-        dom = L.domain(rmin, rmax, lo, hi, lc, bx.linklength, bx.nphalomin)
-        glab = np.empty(len(sub), np.uint32)
-        rc, _ = ctx.link(L.particles(sub), dom, L.opts(L.OPT_NO_HALOS), None, glab, None, 0)
-        L.check(rc)
-        rlab = O.labels_from_ring(nxt)
-        exact = exact and bool(np.array_equal(glab, rlab)) and bool(np.array_equal(ulab, rlab))
+    lc = [bx.box / bx.pside] * 3
+    boxes = []
+    owned_tot, exact, cpu_s = 0, True, 0.0
+    t_wall = time.perf_counter()
+    pending = []
+    nsub = 0
+    while time.perf_counter() - t_wall < budget_s or pending:
+        while len(pending) < 2 * ncores and time.perf_counter() - t_wall < budget_s and nsub < 512:
+            c = rng.uniform(S / 2 + shell + 1, bx.box - S / 2 - shell - 1, 3)
+            lo, hi = c - S / 2 - shell, c + S / 2 + shell
+            sub, _ = subbox(recs_u8, xyz, lo, hi)
+            rmin, rmax = c - S / 2, c + S / 2
+            dom = (list(rmin), list(rmax), list(lo), list(hi), lc, bx.linklength, bx.nphalomin)
+            pending.append((pool.apply_async(_cpu_worker, ((sub["Pos"], dom),)), sub, dom))
+            nsub += 1
+        done = [q for q in pending if q[0].ready()]
+        if not done:
+            time.sleep(0.01)
+            continue
+        for q in done:
+            pending.remove(q)
+            rlab, dt = q[0].get()
+            sub, dom = q[1], q[2]
+            cpu_s += dt
+            p = sub["Pos"].astype(np.float64)
+            owned_tot += int((((p >= dom[0]) & (p < dom[1])).all(1)).sum())
+            glab = np.empty(len(sub), np.uint32)
+            d = L.domain(*dom)
+            rc, _ = ctx.link(L.particles(sub), d, L.opts(L.OPT_NO_HALOS), None, glab, None, 0)
+            L.check(rc)
+            exact = exact and bool(np.array_equal(glab, rlab))
+            boxes.append(len(sub))
+    wall = time.perf_counter() - t_wall
     return {
-        "value": done_owned / done_t if done_t > 0 else None,
+        "value": owned_tot / wall,
         "unit": "owned particles/s",
-        "cores": 1,
+        "cores": ncores,
         "kind": "port",
-        "sample": f"{nbox} interior sub-boxes of {S} Mpc/h (+{shell} Mpc/h shell) of the same "
-                  f"1024^3 realization, {done_owned} owned particles, {done_t:.1f} s; C restatement "
-                  f"of the reference's serial cell sweep + ring merge (the reference itself needs "
-                  f"GSL headers absent from the image)",
+        "sample": f"{len(boxes)} interior sub-boxes of {S} Mpc/h (+{shell} Mpc/h shell) of the same "
+                  f"1024^3 realization, {owned_tot} owned particles, one process per core on "
+                  f"{ncores} cores, {wall:.1f} s wall ({cpu_s:.1f} core-s); C restatement of the "
+                  f"reference's serial cell sweep + ring merge (the reference itself needs GSL "
+                  f"headers absent from the image). Sub-box rates bound the full box from above: "
+                  f"the ring merge costs grow with the cube of clump size",
         "gpu_labels_bit_exact_on_sample": exact,
-        "uf_restatement_particles_per_s": done_owned / uf_t if uf_t > 0 else None,
     }
+
+
+# ---- parity of the timed 1024^3 output --------------------------------------
+def parity_sample(recs_u8, N, lab_dev, halos_u8, nh, bx, O, L, nbox=8, S=30.0, seed=99):
+    """Groups of the full-size GPU output against the oracle: sub-boxes of
+    the linked domain are linked by the oracle (union-find over the reference
+    candidate set); every oracle group whose members all lie more than 2 l
+    inside its sub-box is a complete group of the whole domain and must
+    appear in the GPU labels with exactly the same members.  Halos among
+    them give the halo-property error against compensated FP64 sums."""
+    import torch
+
+    xyz = recs_u8.view(torch.float32).view(-1, 8)[:N, 2:5]
+    lab = lab_dev[:N].to(torch.int64) & 0xFFFFFFFF
+    gsize = torch.bincount(lab, minlength=N)
+    h = halos_u8[: nh * 64].view(torch.int32).view(nh, 16)
+    heads = (h[:, 0].to(torch.int64) & 0xFFFFFFFF).cpu().numpy()
+    hsum = halos_u8[: nh * 64].cpu().numpy().view(L.HALO_DTYPE)
+    head_row = {int(x): k for k, x in enumerate(heads)}
+    rng = np.random.default_rng(seed)
+    l = bx.linklength
+    groups = particles = halos_checked = 0
+    exact = True
+    err = {"n": 0.0, "com": 0.0, "vel": 0.0}
+    for _ in range(nbox):
+        c = rng.uniform(S / 2 + 1, bx.box - S / 2 - 1, 3)
+        lo, hi = c - S / 2, c + S / 2
+        sub, idx = subbox(recs_u8[: N * 32], xyz, lo, hi)
+        od = O.make_domain(lo, hi, lo, hi, [bx.box / bx.pside] * 3, l, 1)
+        olab, _ = O.link_uf(sub["Pos"], od)
+        p = sub["Pos"].astype(np.float64)
+        inner = ((p > lo + 2 * l) & (p < hi - 2 * l)).all(1)
+        # groups with every member inside
+        bad = np.zeros(len(sub), bool)
+        np.logical_or.at(bad, olab, ~inner)
+        ok = ~bad[olab]
+        g_lab = lab[torch.from_numpy(idx).to(lab.device)].cpu().numpy()
+        g_size = gsize[torch.from_numpy(g_lab).to(lab.device)].cpu().numpy()
+        for g in np.unique(olab[ok]):
+            m = np.flatnonzero(olab == g)
+            same = bool((g_lab[m] == g_lab[m[0]]).all() and g_size[m[0]] == len(m))
+            exact = exact and same
+            groups += 1
+            particles += len(m)
+            if len(m) >= bx.nphalomin and same:
+                k = head_row.get(int(g_lab[m[0]]))
+                if k is None:
+                    exact = False
+                    continue
+                hl = np.array([g], np.uint32)
+                sp, sv = O.props_compensated(olab, sub["Pos"], sub["Vel"], hl)
+                n = len(m)
+                err["n"] = max(err["n"], abs(int(hsum["n"][k]) - n) / n)
+                com_g, com_o = hsum["sum_pos"][k] / n, sp[0] / n
+                vel_g, vel_o = hsum["sum_vel"][k] / n, sv[0] / n
+                err["com"] = max(err["com"], float(np.max(np.abs(com_g - com_o) / np.abs(com_o))))
+                err["vel"] = max(err["vel"], float(np.max(np.abs(vel_g - vel_o) /
+                                                          np.maximum(np.abs(vel_o), 1e-30))))
+                halos_checked += 1
+    return {
+        "subboxes": nbox, "side_mpc_h": S, "groups_checked": groups, "particles_checked": particles,
+        "halos_checked": halos_checked, "exact": exact,
+        "halo_err_max_rel": err,
+        "method": "oracle union-find on sub-boxes of the timed call's own input; groups more than "
+                  "2 l inside a sub-box compared member for member with the GPU labels of the "
+                  "whole domain (group sizes from the full label array)",
+    }
+
+
+def pcie_rates(dev):
+    """Measured pinned-host <-> device copy rates (4 GiB each way)."""
+    import torch
+
+    nb = 1 << 32
+    h = torch.empty(nb, dtype=torch.uint8, pin_memory=True)
+    d = torch.empty(nb, dtype=torch.uint8, device=dev)
+    d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d = nb / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    h.copy_(d, non_blocking=True)
+    torch.cuda.synchronize()
+    d2h = nb / (time.perf_counter() - t0)
+    del h, d
+    return h2d, d2h
 
 
 def main():
@@ -273,20 +461,25 @@ def main():
     ap.add_argument("--g", type=int, default=4, help="subvolumes per axis for the batched path")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-volumes", action="store_true", help="skip the volumes block of the box line")
+    ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--workload", default="box", choices=["box", "precision", "volumes", "lightcone"],
-                    help="box: the headline (1024^3, plain vs batched); precision: the same box "
-                         "with 1%% of particles in borderline pairs; volumes: independent "
-                         "periodic volumes with log-normal counts")
+                    help="box: the headline (1024^3, plain vs batched, plus the 4096-volume batch); "
+                         "precision: the same box with 1%% of particles in borderline pairs; "
+                         "volumes: independent periodic volumes with log-normal counts; "
+                         "lightcone: 64 shells, variable linking length")
     ap.add_argument("--nvol", type=int, default=4096)
     ap.add_argument("--lc-n", type=int, default=400_000_000, help="lightcone particles")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 4 GiB device-to-device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the FoF domains shard "
-              "by rank with no GPU-to-GPU traffic, so value at N GPUs is N x the 1-GPU value",
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build measures one "
+              "GPU's share of the domain decomposition; there is no multi-GPU run",
               file=sys.stderr)
         sys.exit(2)
+
+    pool, ncores = (None, 0) if (args.no_cpu or args.workload != "box") else cpu_pool()
 
     import torch
 
@@ -305,10 +498,13 @@ def main():
         torch.cuda.synchronize()
         log("copy probe: 4 GiB device-to-device copy done")
         del a, b
+    ctx = L.Context(0)
     if args.workload == "volumes":
-        return run_volumes(args, torch, L, W, O)
+        print(json.dumps(run_volumes(args, torch, L, W, O, ctx)), flush=True)
+        return
     if args.workload == "lightcone":
-        return run_lightcone(args, torch, L, W, O)
+        print(json.dumps(run_lightcone(args, torch, L, W, O, ctx)), flush=True)
+        return
     if args.workload == "precision":
         bx = W.precision(pside=args.pside)
     else:
@@ -330,49 +526,92 @@ def main():
     t0 = time.perf_counter()
     D1, N1, V1 = build(1)
     DB, NB, VB = build(args.g)
+    del recs  # interior sub-boxes of D1 hold the same particles
+    torch.cuda.empty_cache()
     log(f"ghost shells: plain {N1}, batched {NB} ({args.g}^3 volumes) in {time.perf_counter() - t0:.1f}s")
 
-    ctx = L.Context(0)
     Nmax = max(N1, NB)
     nxt = torch.empty(Nmax, dtype=torch.int32, device=dev)
     lab = torch.empty(Nmax, dtype=torch.int32, device=dev)
     cap = Nmax // 20 + 16
     hal = torch.empty(cap * 64, dtype=torch.uint8, device=dev)
     hoff = np.zeros(len(VB) + 1, np.uint64)
-    flags = L.OPT_OUT_DEVICE | L.OPT_TIMING
+    # timed calls: autotuned configuration, replayed as a captured graph;
+    # per-phase times come from one eager call with the same configuration
+    flags = L.OPT_OUT_DEVICE | L.OPT_AUTOTUNE | L.OPT_GRAPH
     st1, stB = L.Stats(), L.Stats()
     P1, PB = L.particles(D1, n=N1), L.particles(DB, n=NB)
     dom1 = V1[0].domain
     nh1 = [0]
 
-    def step_plain():
-        rc, nh = ctx.link(P1, dom1, L.opts(flags), nxt, lab, hal, cap, st1)
+    def step_plain(fl=flags, st=None):
+        rc, nh = ctx.link(P1, dom1, L.opts(fl), nxt, lab, hal, cap, st)
         L.check(rc, "plain")
         nh1[0] = nh
 
-    def step_batch():
-        rc = ctx.link_batch(PB, VB, L.opts(flags), nxt, lab, hal, cap, hoff, stB)
+    def step_batch(fl=flags, st=None):
+        rc = ctx.link_batch(PB, VB, L.opts(fl), nxt, lab, hal, cap, hoff, st)
         L.check(rc, "batched")
 
+    t0 = time.perf_counter()
+    step_plain()
+    torch.cuda.synchronize()
+    log(f"plain: autotuned in {time.perf_counter() - t0:.1f}s")
     t_plain = run_steps(step_plain, args.warmup, args.steps)
-    log(f"plain: {t_plain * 1e3:.1f} ms/step  phases {dict(zip(L.PHASES, [round(x, 2) for x in st1.phase_ms]))}")
+    step_plain(L.OPT_OUT_DEVICE | L.OPT_AUTOTUNE | L.OPT_TIMING, st1)
+    log(f"plain: {t_plain * 1e3:.1f} ms/step (T={st1.dense_threshold})  phases "
+        f"{dict(zip(L.PHASES, [round(x, 2) for x in st1.phase_ms]))}")
     fp1 = group_fingerprints(D1, N1, lab, hal, nh1[0])
+    parity = None
+    if not args.no_parity:
+        t0 = time.perf_counter()
+        parity = parity_sample(D1, N1, lab, hal, nh1[0], bx, O, L)
+        log(f"parity sample of the timed 1024^3 output ({time.perf_counter() - t0:.1f}s): {parity}")
+    cpu = None
+    if pool is not None:
+        cpu = cpu_baseline(D1, bx, pool, ncores, ctx, L, args.cpu_budget)
+        log("cpu baseline:", cpu)
+    del D1
+    torch.cuda.empty_cache()
+
+    t0 = time.perf_counter()
+    step_batch()
+    torch.cuda.synchronize()
+    log(f"batched: autotuned in {time.perf_counter() - t0:.1f}s")
     t_batch = run_steps(step_batch, args.warmup, args.steps)
-    log(f"batched: {t_batch * 1e3:.1f} ms/step  phases {dict(zip(L.PHASES, [round(x, 2) for x in stB.phase_ms]))}")
+    stG = L.Stats()
+    step_batch(flags, stG)  # the timed configuration (graph replay)
+    # phases and per-kernel counters: one eager single-pipeline call with the
+    # tuned dense threshold and grid
+    L.check(ctx.link_batch(PB, VB, L.opts(L.OPT_OUT_DEVICE | L.OPT_TIMING, 0.0, stG.dense_threshold, 1,
+                                          stG.big_grid), nxt, lab, hal, cap, hoff, stB), "batched")
+    log(f"batched: {t_batch * 1e3:.1f} ms/step (T={stG.dense_threshold}, pipelines={stG.pipelines}, "
+        f"grid={stG.big_grid}, graph={stG.graph})  eager single-pipeline phases "
+        f"{dict(zip(L.PHASES, [round(x, 2) for x in stB.phase_ms]))}")
     nhB = int(hoff[-1])
     fpB = group_fingerprints(DB, NB, lab, hal, nhB, [int(v.offset) for v in VB], [int(v.n) for v in VB],
                              hoff)
     same_groups = bool(fp1.shape == fpB.shape and torch.equal(fp1, fpB))
     log(f"owned halos plain {fp1.shape[0]} batched {fpB.shape[0]} identical membership: {same_groups}")
 
-    roof = roofline_link(stB, L, DB, VB,
-                         "box_1024" if args.workload == "box" and args.pside == 1024 and args.g == 4 else None)
+    cellsB = L.plan(PB, VB).cells
+    wl = "box_1024" if args.workload == "box" and args.pside == 1024 and args.g == 4 else None
+    roof = roofline(stB, L, DB, VB, NB, cellsB, wl)
     log("roofline:", roof)
+    h2d, d2h = pcie_rates(dev)
+    pcie = {"h2d_GBs": h2d / 1e9, "d2h_GBs": d2h / 1e9,
+            "input_bytes": N1 * 32, "output_bytes": N1 * 8 + nh1[0] * 64,
+            "h2d_ms": N1 * 32 / h2d * 1e3, "d2h_ms": (N1 * 8 + nh1[0] * 64) / d2h * 1e3,
+            "note": "measured pinned copy rates; a host-buffer call adds these copies of the "
+                    "plain domain's input and outputs (ring + labels + halos) to ms_per_step"}
+    log("pcie:", pcie)
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(recs, bx, args.cpu_budget, ctx, L, O)
-        log("cpu baseline:", cpu)
+    del nxt, hal, DB, lab
+    torch.cuda.empty_cache()
+    vol_res = None
+    if not args.no_volumes and args.workload == "box":
+        vol_res = run_volumes(args, torch, L, W, O, ctx)
+        log("volumes:", vol_res)
 
     value = n / t_batch
     res = {
@@ -396,20 +635,41 @@ def main():
             "particles_with_shells": {"plain": N1, "batched": NB},
         },
         "plain": {"value": n / t_plain, "ms_per_step": t_plain * 1e3,
-                  "phase_ms": {k: round(v, 3) for k, v in zip(L.PHASES, st1.phase_ms)}},
+                  "dense_threshold": st1.dense_threshold,
+                  "phase_ms_eager": {k: round(v, 3) for k, v in zip(L.PHASES, st1.phase_ms)}},
         "batched": {"value": value, "ms_per_step": t_batch * 1e3,
-                    "phase_ms": {k: round(v, 3) for k, v in zip(L.PHASES, stB.phase_ms)}},
+                    "dense_threshold": stG.dense_threshold, "pipelines": stG.pipelines,
+                    "big_grid": stG.big_grid, "graph_replay": stG.graph == 1,
+                    "phase_ms_eager": {k: round(v, 3) for k, v in zip(L.PHASES, stB.phase_ms)}},
+        "launch": "timed calls use FOF_OPT_AUTOTUNE (dense threshold, pipelines and dense-kernel "
+                  "grid chosen per workload shape by timing candidates once) and FOF_OPT_GRAPH "
+                  "(the call's kernels captured once and replayed as one HIP graph); phase times "
+                  "come from one eager single-pipeline call with the tuned dense threshold "
+                  "and grid",
         "speedup_batched_vs_plain": t_plain / t_batch,
         "groups_identical_plain_vs_batched": same_groups,
+        "groups_exact_vs_oracle_on_sample": parity,
+        "halo_err": {"config": "1024^3 box, halos of the parity sample vs compensated FP64 sums",
+                     **(parity["halo_err_max_rel"] if parity else {})},
         "stats": {"pair_tests": stB.pair_tests, "rechecks": stB.rechecks, "ndense": stB.ndense,
                   "nhalos_owned": stB.nhalos_owned, "ngroups": stB.ngroups},
         "roofline": roof,
+        "pcie": pcie,
         "cpu_baseline": cpu,
+        "volumes": vol_res,
     }
     print(json.dumps(res), flush=True)
+    if pool is not None:
+        pool.terminate()
+    ok = same_groups and (parity is None or parity["exact"]) and \
+        (cpu is None or cpu["gpu_labels_bit_exact_on_sample"]) and \
+        (vol_res is None or vol_res["sampled_volumes_bit_exact"])
+    if not ok:
+        log("FAILED: a parity check of this run did not hold")
+        sys.exit(1)
 
 
-def run_volumes(args, torch, L, W, O):
+def run_volumes(args, torch, L, W, O, ctx):
     """Independent periodic volumes (log-normal counts, median 6e4, clipped to
     [1e3, 2e6], b ~ U(0.15, 0.25)): one batched call for all of them, and one
     call per volume for per-volume latencies; a sample of volumes is checked
@@ -422,7 +682,6 @@ def run_volumes(args, torch, L, W, O):
     dev = torch.device("cuda:0")
     D = torch.from_numpy(P.view(np.uint8).reshape(-1)).to(dev)
     N = len(P)
-    ctx = L.Context(0)
     nxt = torch.empty(N, dtype=torch.int32, device=dev)
     lab = torch.empty(N, dtype=torch.int32, device=dev)
     cap = N // 20 + 16 * len(vols)
@@ -437,14 +696,27 @@ def run_volumes(args, torch, L, W, O):
 
     t_batch = run_steps(step, args.warmup, args.steps)
     labs = lab.cpu().numpy().view(np.uint32)
+    hals = hal.cpu().numpy().view(L.HALO_DTYPE)
     rng = np.random.default_rng(1)
     exact = True
+    err = {"n": 0.0, "com": 0.0, "vel": 0.0}
     for k in rng.choice(len(vols), size=min(12, len(vols)), replace=False):
         v = vols[k]
         a, b = int(v.offset), int(v.offset + v.n)
         d = v.domain
         od = O.make_domain(d.rmin, d.rmax, d.rmin_buff, d.rmax_buff, d.lcell, d.linklength, d.nphalomin)
-        exact = exact and bool(np.array_equal(labs[a:b], O.link_uf(P["Pos"][a:b], od)[0]))
+        olab = O.link_uf(P["Pos"][a:b], od)[0]
+        exact = exact and bool(np.array_equal(labs[a:b], olab))
+        oh = O.retrieve(O.ring_from_labels(olab), P["Pos"][a:b], P["Vel"][a:b], od)
+        gh = hals[int(hoff[k]):int(hoff[k + 1])]
+        exact = exact and len(gh) == len(oh) and bool(np.array_equal(gh["head"], oh["head"]))
+        if len(oh) and len(gh) == len(oh):
+            sp, sv = O.props_compensated(olab, P["Pos"][a:b], P["Vel"][a:b], oh["head"])
+            nn = oh["n"][:, None].astype(np.float64)
+            err["n"] = max(err["n"], float(np.max(np.abs(gh["n"] - oh["n"]) / oh["n"])))
+            err["com"] = max(err["com"], float(np.max(np.abs(gh["sum_pos"] / nn - sp / nn) / np.abs(sp / nn))))
+            err["vel"] = max(err["vel"], float(np.max(np.abs(gh["sum_vel"] / nn - sv / nn) /
+                                                      np.maximum(np.abs(sv / nn), 1e-30))))
     # per-volume latency: one synchronous call per volume on device-resident input
     lat = []
     base = D.data_ptr()
@@ -469,17 +741,20 @@ def run_volumes(args, torch, L, W, O):
                    "particles_with_shells": N},
         "batched_phase_ms": {k: round(x, 3) for k, x in zip(L.PHASES, st.phase_ms)},
         "per_volume_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                                  "max": float(lat.max()), "sum": float(lat.sum())},
+                                  "max": float(lat.max()), "sum": float(lat.sum()),
+                                  "method": "one fof_link_gpu call per volume on device-resident "
+                                            "input, host wall clock around the call"},
         "speedup_batched_vs_one_call_per_volume": float(lat.sum() / 1e3 / t_batch),
         "sampled_volumes_bit_exact": exact,
-        "roofline": roofline_link(st, L, D, vols),
+        "halo_err_max_rel": err,
         "stats": {"pair_tests": st.pair_tests, "rechecks": st.rechecks, "ndense": st.ndense,
                   "nhalos_owned": st.nhalos_owned, "sparse": st.sparse},
     }
-    print(json.dumps(res), flush=True)
+    del D, nxt, lab, hal
+    return res
 
 
-def run_lightcone(args, torch, L, W, O):
+def run_lightcone(args, torch, L, W, O, ctx):
     """Full-sky lightcone to 3000 Mpc/h (flat LCDM, om 0.31), n(r) ~ (1+z)^-1.5,
     l(r) = 0.2 n(r)^(-1/3) as a table on r^2 (home-cell rule), 64 radial
     shells with radial ghosts linked in one batched call (occupied-cell
@@ -497,7 +772,6 @@ def run_lightcone(args, torch, L, W, O):
     torch.cuda.synchronize()
     N = D.numel() // 32
     log(f"lightcone: {len(vols)} shells, {N} particles with ghosts in {time.perf_counter() - t0:.1f}s")
-    ctx = L.Context(0)
     nxt = torch.empty(N, dtype=torch.int32, device=dev)
     lab = torch.empty(N, dtype=torch.int32, device=dev)
     cap = N // 20 + 16 * len(vols)
@@ -514,16 +788,28 @@ def run_lightcone(args, torch, L, W, O):
     log(f"lightcone: {t_batch * 1e3:.1f} ms/step  phases "
         f"{dict(zip(L.PHASES, [round(x, 2) for x in st.phase_ms]))}")
     labs = lab.cpu().numpy().view(np.uint32)
+    hals = hal.cpu().numpy().view(L.HALO_DTYPE)
     r2max, tab = lc.varlink_table()
     ovl = O.varlink([0.0, 0.0, 0.0], r2max, tab)
     exact = True
+    err = {"n": 0.0, "com": 0.0, "vel": 0.0}
     for k in (0, 1):
         v = vols[k]
         a, b = int(v.offset), int(v.offset + v.n)
         d = v.domain
         od = O.make_domain(d.rmin, d.rmax, d.rmin_buff, d.rmax_buff, d.lcell, d.linklength, d.nphalomin)
         Qk = D[a * 32:b * 32].cpu().numpy().view(L.PART_DTYPE)
-        exact = exact and bool(np.array_equal(labs[a:b], O.link_uf(Qk["Pos"], od, ovl)[0]))
+        olab = O.link_uf(Qk["Pos"], od, ovl)[0]
+        exact = exact and bool(np.array_equal(labs[a:b], olab))
+        oh = O.retrieve(O.ring_from_labels(olab), Qk["Pos"], Qk["Vel"], od)
+        gh = hals[int(hoff[k]):int(hoff[k + 1])]
+        if len(oh) and len(gh) == len(oh):
+            sp, sv = O.props_compensated(olab, Qk["Pos"], Qk["Vel"], oh["head"])
+            nn = oh["n"][:, None].astype(np.float64)
+            err["com"] = max(err["com"], float(np.max(np.abs(gh["sum_pos"] / nn - sp / nn) /
+                                                      np.maximum(np.abs(sp / nn), 1e-30))))
+            err["vel"] = max(err["vel"], float(np.max(np.abs(gh["sum_vel"] / nn - sv / nn) /
+                                                      np.maximum(np.abs(sv / nn), 1e-30))))
     log(f"lightcone: shells 0-1 bit-exact vs oracle: {exact}")
     res = {
         "metric": "FoF particles/sec, variable linking length lightcone, 64 shells batched",
@@ -537,10 +823,11 @@ def run_lightcone(args, torch, L, W, O):
                                f"{lc.nshell} shells, l(r) = 0.2 n(r)^-1/3", "particles_with_ghosts": N},
         "phase_ms": {k: round(x, 3) for k, x in zip(L.PHASES, st.phase_ms)},
         "shells_0_1_bit_exact": exact,
+        "halo_err_max_rel": err,
         "stats": {"pair_tests": st.pair_tests, "rechecks": st.rechecks, "ndense": st.ndense,
                   "nhalos_owned": st.nhalos_owned, "sparse": st.sparse, "ncells": st.ncells},
     }
-    print(json.dumps(res), flush=True)
+    return res
 
 
 if __name__ == "__main__":

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "libcmfof.so")
 
 ABI_VERSION = 1
 MEM_HOST, MEM_DEVICE = 0, 1
-OPT_TIMING, OPT_OUT_DEVICE, OPT_NO_RING, OPT_NO_HALOS, OPT_SPARSE = 0x1, 0x2, 0x4, 0x8, 0x20
+OPT_TIMING, OPT_OUT_DEVICE, OPT_NO_RING, OPT_NO_HALOS = 0x1, 0x2, 0x4, 0x8
+OPT_GRAPH, OPT_SPARSE, OPT_AUTOTUNE = 0x10, 0x20, 0x40
 
 E_ARG, E_DOMAIN, E_RANGE, E_NOMEM, E_HIP, E_NODEV = -1, -2, -3, -4, -5, -6
 
@@ -27,6 +28,8 @@ ABI_SYMBOLS = (
     "fof_ctx_destroy",
     "fof_link_gpu",
     "fof_link_gpu_batch",
+    "fof_link_gpu_ex",
+    "fof_plan",
     "fof_strerror",
     "fof_device_info",
     "fof_gen_host",
@@ -78,6 +81,8 @@ class Opts(C.Structure):
         ("flags", C.c_uint32),
         ("band", C.c_double),
         ("dense_threshold", C.c_uint32),
+        ("pipelines", C.c_uint32),
+        ("big_grid", C.c_uint32),
         ("reserved", C.c_uint32),
     ]
 
@@ -128,12 +133,67 @@ class Stats(C.Structure):
         ("dense_threshold", C.c_uint32),
         ("phase_ms", C.c_double * NPHASE),
         ("link_kernel_ms", C.c_double),
+        ("pair_tests_kernel", C.c_uint64 * 3),
+        ("pairs_joined", C.c_uint64),
+        ("pipelines", C.c_uint32),
+        ("graph", C.c_uint32),
+        ("big_grid", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "phase_ms"}
+        d = {k: getattr(self, k) for k, _ in self._fields_
+             if k not in ("phase_ms", "pair_tests_kernel")}
         d["phase_ms"] = dict(zip(PHASES, self.phase_ms))
+        d["pair_tests_kernel"] = dict(zip(LINK_KERNELS, self.pair_tests_kernel))
         return d
+
+
+# pair_tests_kernel order
+LINK_KERNELS = ("k_link_sparse", "k_link_dense", "k_link_dense_wave")
+
+
+class Moments(C.Structure):
+    _fields_ = [
+        ("pos2", C.c_double * 6),
+        ("vel2", C.c_double * 6),
+        ("cpos2", C.c_double * 6),
+        ("cvel2", C.c_double * 6),
+    ]
+
+
+MOMENTS_DTYPE = np.dtype([("pos2", "<f8", 6), ("vel2", "<f8", 6), ("cpos2", "<f8", 6),
+                          ("cvel2", "<f8", 6)])
+assert MOMENTS_DTYPE.itemsize == C.sizeof(Moments) == 192
+
+
+class Outputs(C.Structure):
+    _fields_ = [
+        ("version", C.c_uint32),
+        ("location", C.c_uint32),
+        ("next", C.c_void_p),
+        ("label", C.c_void_p),
+        ("minid", C.c_void_p),
+        ("halos", C.c_void_p),
+        ("moments", C.c_void_p),
+        ("halo_cap", C.c_uint64),
+        ("halo_offsets", C.c_void_p),
+        ("nhalos", C.c_uint64),
+        ("stats", C.c_void_p),
+    ]
+
+
+class Plan(C.Structure):
+    _fields_ = [
+        ("version", C.c_uint32),
+        ("sparse", C.c_uint32),
+        ("n", C.c_uint64),
+        ("cells", C.c_uint64),
+        ("grid_cells", C.c_double),
+        ("workspace_bytes", C.c_uint64),
+        ("dense_threshold", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
 
 
 class Volume(C.Structure):
@@ -222,6 +282,13 @@ def lib():
             vp, vp, vp, u64, vp, C.POINTER(Stats),
         ]
         L.fof_link_gpu_batch.restype = C.c_int
+        L.fof_link_gpu_ex.argtypes = [
+            vp, C.POINTER(Particles), C.POINTER(Volume), u32, C.POINTER(Opts), C.POINTER(Outputs),
+        ]
+        L.fof_link_gpu_ex.restype = C.c_int
+        L.fof_plan.argtypes = [C.POINTER(Particles), C.POINTER(Volume), u32, C.POINTER(Opts),
+                               C.POINTER(Plan)]
+        L.fof_plan.restype = C.c_int
         L.fof_strerror.argtypes = [C.c_int]
         L.fof_strerror.restype = C.c_char_p
         L.fof_device_info.argtypes = [C.c_int, C.c_char_p, C.c_size_t]
@@ -304,13 +371,24 @@ class VarLinkTable:
         self.s.l2tab = self.tab.ctypes.data
 
 
-def opts(flags=0, band=0.0, dense_threshold=0) -> Opts:
+def opts(flags=0, band=0.0, dense_threshold=0, pipelines=0, big_grid=0) -> Opts:
     o = Opts()
     o.version = ABI_VERSION
     o.flags = flags
     o.band = band
     o.dense_threshold = dense_threshold
+    o.pipelines = pipelines
+    o.big_grid = big_grid
     return o
+
+
+def plan(parts: Particles, vols, o: Opts | None = None) -> Plan:
+    """fof_plan: the addressing, grid sizes and workspace of a call (host only)."""
+    arr = (Volume * len(vols))(*vols)
+    p = Plan()
+    check(lib().fof_plan(C.byref(parts), arr, len(vols), C.byref(o) if o else None, C.byref(p)),
+          "fof_plan")
+    return p
 
 
 def device_info(dev=0) -> str:
@@ -347,6 +425,30 @@ class Context:
             C.byref(nh), C.byref(stats) if stats else None,
         )
         return rc, nh.value
+
+    def link_ex(self, parts: Particles, vols, o: Opts | None = None, next_out=None,
+                label_out=None, minid_out=None, halos=None, moments=None, halo_cap=0,
+                halo_offsets=None, stats: Stats | None = None, location=None):
+        """fof_link_gpu_ex: returns (rc, nhalos).  `location` defaults to where
+        the first given output array lives."""
+        arr = (Volume * len(vols))(*vols)
+        out = Outputs()
+        out.version = ABI_VERSION
+        if location is None:
+            location = MEM_HOST
+            for x in (next_out, label_out, minid_out, halos, moments):
+                if x is not None:
+                    location = ptr(x)[1]
+                    break
+        out.location = location
+        out.next, out.label, out.minid = ptr(next_out)[0], ptr(label_out)[0], ptr(minid_out)[0]
+        out.halos, out.moments = ptr(halos)[0], ptr(moments)[0]
+        out.halo_cap = halo_cap
+        out.halo_offsets = ptr(halo_offsets)[0]
+        out.stats = C.cast(C.pointer(stats), C.c_void_p) if stats is not None else None
+        rc = lib().fof_link_gpu_ex(self.h, C.byref(parts), arr, len(vols),
+                                   C.byref(o) if o else None, C.byref(out))
+        return rc, out.nhalos
 
     def link_batch(self, parts: Particles, vols, o: Opts | None = None, next_out=None,
                    label_out=None, halos=None, halo_cap=0, halo_offsets=None,

@@ -1,21 +1,19 @@
 // fof_api.hip -- C ABI of the FoF engine: contexts, argument checks, errors.
-// Argument and domain checks run on the host before any device work, so a
-// bad call fails the same way with or without a GPU.
+// Argument checks run on the host before any device work, so a bad call
+// fails the same way with or without a GPU.  The domain check (every
+// particle inside [rmin_buff, rmax_buff]) runs on the host for host-resident
+// particles, before any device work, and inside the first kernel for
+// device-resident ones (reported when the call ends).
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <new>
+#include <thread>
+#include <vector>
 
 #include "fof_internal.h"
-
-extern "C" int fof_engine_init(fof_ctx *ctx);
-extern "C" void fof_engine_free(fof_ctx *ctx);
-extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1 *vols,
-                              uint32_t nvol, const fof_opts_v1 *opts, uint32_t *next_out,
-                              uint32_t *label_out, fof_halo_v1 *halos, uint64_t halo_cap,
-                              uint64_t *halo_offsets, uint64_t *nhalos_out,
-                              fof_stats_v1 *stats);
 
 static thread_local char g_last_error[256];
 
@@ -55,6 +53,43 @@ int check_opts(const fof_opts_v1 *o) {
   if (!o) return 0;
   if (o->version != FOF_ABI_VERSION) return FOF_E_ARG;
   if (o->band != 0 && !(o->band >= 1e-6 && o->band <= 0.1)) return FOF_E_ARG;
+  return 0;
+}
+
+// Domain check of host-resident particles, before any device work: every
+// position inside its volume's [rmin_buff, rmax_buff] (the bounds of
+// src_v2/read_data.c:810-812; NaN fails the comparisons).  Split over up to
+// 16 host threads.
+int check_domain_host(const fof_particles_v1 *P, const fof_volume_v1 *vols, uint32_t nvol) {
+  if (P->location != FOF_MEM_HOST || P->n == 0) return 0;
+  const uint8_t *base = static_cast<const uint8_t *>(P->base);
+  const uint64_t N = P->n;
+  const unsigned nth =
+      (unsigned)std::max<uint64_t>(1, std::min<uint64_t>({16u, std::thread::hardware_concurrency(), N / 65536 + 1}));
+  std::vector<int> bad(nth, 0);
+  auto work = [&](unsigned t) {
+    const uint64_t a = N * t / nth, e = N * (t + 1) / nth;
+    uint32_t v = 0;
+    while (v + 1 < nvol && vols[v].offset + vols[v].n <= a) v++;
+    for (uint64_t p = a; p < e; p++) {
+      while (p >= vols[v].offset + vols[v].n) v++;
+      const fof_domain_v1 &D = vols[v].domain;
+      const float *q = reinterpret_cast<const float *>(base + p * P->stride + P->off_pos);
+      for (int d = 0; d < 3; d++) {
+        const double x = q[d];
+        if (!(x >= D.rmin_buff[d] && x <= D.rmax_buff[d])) {
+          bad[t] = 1;
+          return;
+        }
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < nth; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto &x : th) x.join();
+  for (int x : bad)
+    if (x) return FOF_E_DOMAIN;
   return 0;
 }
 
@@ -123,10 +158,8 @@ static int resolve_ctx(fof_ctx **ctx) {
   return 0;
 }
 
-int fof_link_gpu_batch(fof_ctx *ctx, const fof_particles_v1 *particles,
-                       const fof_volume_v1 *vols, uint32_t nvol, const fof_opts_v1 *opts,
-                       uint32_t *next_out, uint32_t *label_out, fof_halo_v1 *halos,
-                       uint64_t halo_cap, uint64_t *halo_offsets, fof_stats_v1 *stats) {
+static int check_batch(const fof_particles_v1 *particles, const fof_volume_v1 *vols,
+                       uint32_t nvol, const fof_opts_v1 *opts) {
   int rc = check_particles(particles);
   if (rc) return rc;
   if ((rc = check_opts(opts))) return rc;
@@ -137,29 +170,69 @@ int fof_link_gpu_batch(fof_ctx *ctx, const fof_particles_v1 *particles,
     total += vols[v].n;
   }
   if (total != particles->n) return FOF_E_ARG;
+  return 0;
+}
+
+int fof_link_gpu_ex(fof_ctx *ctx, const fof_particles_v1 *particles, const fof_volume_v1 *vols,
+                    uint32_t nvol, const fof_opts_v1 *opts, fof_outputs_v1 *out) {
+  if (!out || out->version != FOF_ABI_VERSION) return FOF_E_ARG;
+  if (out->location != FOF_MEM_HOST && out->location != FOF_MEM_DEVICE) return FOF_E_ARG;
+  out->nhalos = 0;
+  int rc = check_batch(particles, vols, nvol, opts);
+  if (rc) return rc;
+  if (out->minid && particles->off_id < 0) return FOF_E_ARG;
+  if ((rc = check_domain_host(particles, vols, nvol))) return rc;
   if ((rc = resolve_ctx(&ctx))) return rc;
-  uint64_t nh = 0;
-  return fof_engine_run(ctx, particles, vols, nvol, opts, next_out, label_out, halos, halo_cap,
-                        halo_offsets, &nh, stats);
+  return fof_engine_run(ctx, particles, vols, nvol, opts, out);
+}
+
+static fof_outputs_v1 legacy_outputs(const fof_opts_v1 *opts, uint32_t *next_out,
+                                     uint32_t *label_out, fof_halo_v1 *halos, uint64_t halo_cap,
+                                     uint64_t *halo_offsets, fof_stats_v1 *stats) {
+  fof_outputs_v1 o;
+  memset(&o, 0, sizeof(o));
+  o.version = FOF_ABI_VERSION;
+  o.location = (opts && (opts->flags & FOF_OPT_OUT_DEVICE)) ? FOF_MEM_DEVICE : FOF_MEM_HOST;
+  o.next = next_out;
+  o.label = label_out;
+  o.halos = halos;
+  o.halo_cap = halo_cap;
+  o.halo_offsets = halo_offsets;
+  o.stats = stats;
+  return o;
+}
+
+int fof_link_gpu_batch(fof_ctx *ctx, const fof_particles_v1 *particles,
+                       const fof_volume_v1 *vols, uint32_t nvol, const fof_opts_v1 *opts,
+                       uint32_t *next_out, uint32_t *label_out, fof_halo_v1 *halos,
+                       uint64_t halo_cap, uint64_t *halo_offsets, fof_stats_v1 *stats) {
+  fof_outputs_v1 o = legacy_outputs(opts, next_out, label_out, halos, halo_cap, halo_offsets, stats);
+  return fof_link_gpu_ex(ctx, particles, vols, nvol, opts, &o);
 }
 
 int fof_link_gpu(fof_ctx *ctx, const fof_particles_v1 *particles, const fof_domain_v1 *domain,
                  const fof_opts_v1 *opts, uint32_t *next_out, uint32_t *label_out,
                  fof_halo_v1 *halos, uint64_t halo_cap, uint64_t *nhalos, fof_stats_v1 *stats) {
+  if (nhalos) *nhalos = 0;
   int rc = check_particles(particles);
   if (rc) return rc;
-  if ((rc = check_opts(opts))) return rc;
   if (!domain) return FOF_E_ARG;
-  if ((rc = resolve_ctx(&ctx))) return rc;
   fof_volume_v1 v;
   v.offset = 0;
   v.n = particles->n;
   v.domain = *domain;
-  uint64_t nh = 0;
-  rc = fof_engine_run(ctx, particles, &v, 1, opts, next_out, label_out, halos, halo_cap,
-                      nullptr, &nh, stats);
-  if (nhalos) *nhalos = nh;
+  fof_outputs_v1 o = legacy_outputs(opts, next_out, label_out, halos, halo_cap, nullptr, stats);
+  rc = fof_link_gpu_ex(ctx, particles, &v, 1, opts, &o);
+  if (nhalos) *nhalos = o.nhalos;
   return rc;
+}
+
+int fof_plan(const fof_particles_v1 *particles, const fof_volume_v1 *vols, uint32_t nvol,
+             const fof_opts_v1 *opts, fof_plan_v1 *out) {
+  if (!out) return FOF_E_ARG;
+  int rc = check_batch(particles, vols, nvol, opts);
+  if (rc) return rc;
+  return fof_engine_plan(particles, vols, nvol, opts, out);
 }
 
 int fof_device_info(int device, char *buf, size_t len) {

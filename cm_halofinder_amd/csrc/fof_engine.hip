@@ -36,6 +36,11 @@
 #include <math.h>
 #include <string.h>
 
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "fof_internal.h"
@@ -365,29 +370,50 @@ __device__ __forceinline__ uint32_t uf_join(uint32_t *par, uint32_t rx, uint32_t
   }
 }
 
-// Statistics counters (pair tests, FP64 rechecks, groups, owned halos) are
-// spread over kCtrSlots copies, one 128-byte line each, picked by workgroup:
-// a million waves adding to one address serialise at one L2 channel.  The
-// host sums the copies.
+// Statistics counters are spread over kCtrSlots copies, one 128-byte line
+// each, picked by workgroup: a million waves adding to one address serialise
+// at one L2 channel.  The host sums the copies.  Counter k of a slot:
+constexpr int kCtrTestsSparse = 0;  // pair tests in k_link_sparse
+constexpr int kCtrRechecks = 1;     // tests re-decided in FP64 (all kernels)
+constexpr int kCtrGroups = 2;       // groups of any size
+constexpr int kCtrOwned = 3;        // owned halos
+constexpr int kCtrTestsBig = 4;     // pair tests in k_link_dense, 256 threads
+constexpr int kCtrTestsWave = 5;    // pair tests in k_link_dense, one wave
+constexpr int kCtrJoined = 6;       // candidate run pairs found already joined
 constexpr int kCtrSlots = 64;
 constexpr int kCtrStride = 16;  // u64 per slot
 __device__ __forceinline__ void ctr_add(unsigned long long *ctr, int k, unsigned long long v) {
   atomicAdd(ctr + (blockIdx.x & (kCtrSlots - 1)) * kCtrStride + k, v);
 }
 
-__device__ __forceinline__ void add_counters(unsigned long long *ctr, uint32_t tests,
-                                             uint32_t rechecks) {
-  uint64_t t = tests, r = rechecks;
+__device__ __forceinline__ void add_counters(unsigned long long *ctr, int ktests, uint32_t tests,
+                                             uint32_t rechecks, uint32_t joined = 0) {
+  uint64_t t = tests, r = rechecks, j = joined;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     t += __shfl_down(t, o, 64);
     r += __shfl_down(r, o, 64);
+    j += __shfl_down(j, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
-    if (t) ctr_add(ctr, 0, t);
-    if (r) ctr_add(ctr, 1, r);
+    if (t) ctr_add(ctr, ktests, t);
+    if (r) ctr_add(ctr, kCtrRechecks, r);
+    if (j) ctr_add(ctr, kCtrJoined, j);
   }
 }
+
+// Device-side counts and flags of a call (u32 words of the `small` block);
+// the counters follow at word kSmCtr.
+constexpr int kSmErr = 0;       // bit 0: particle outside [rmin_buff, rmax_buff]
+constexpr int kSmNDense = 1;    // dense cells below kBigCell particles
+constexpr int kSmM = 2;         // members of kept groups (label-sorted list)
+constexpr int kSmH = 3;         // halos
+constexpr int kSmOverflow = 4;  // run-table overflow
+constexpr int kSmU = 5;         // occupied cells (sparse addressing)
+constexpr int kSmNBig = 7;      // dense cells from kBigCell particles
+constexpr int kSmPairs = 8;     // (index, label) pairs of group members
+constexpr int kSmNtB = 10;      // tiles of the fine partition pass
+constexpr int kSmCtr = 64;
 
 // Dense cells with at least this many particles are linked by a 256-thread
 // workgroup each, smaller ones by a single wave each.
@@ -574,7 +600,41 @@ __global__ void k_pa_starts(uint32_t Bc, uint32_t ntiles, const uint32_t *__rest
 }
 
 // B: tiles inside coarse buckets.  tinfo[t] = coarse bucket of tile t;
-// t0[c] = first tile of bucket c (t0[Bc] = number of tiles).
+// t0[c] = first tile of bucket c (t0[Bc] = number of tiles, also left in
+// *ntb).  Built on the device from the coarse bucket starts; the pass-B
+// kernels are launched for the most tiles the buckets can need
+// (ntiles_A + Bc) and the blocks past t0[Bc] do no work.
+__global__ __launch_bounds__(kMaxCoarse) void k_pb_tiles(uint32_t Bc, uint32_t ntmax,
+                                                         const uint32_t *__restrict__ cstart,
+                                                         uint32_t *__restrict__ t0,
+                                                         uint32_t *__restrict__ tinfo,
+                                                         uint32_t *__restrict__ ntb) {
+  __shared__ uint32_t ws[kMaxCoarse / 64];
+  const uint32_t c = threadIdx.x, lane = c & 63, w = c >> 6;
+  const uint32_t nt = c < Bc ? (cstart[c + 1] - cstart[c] + kPartTile - 1) / kPartTile : 0u;
+  uint32_t inc = nt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) ws[w] = inc;
+  __syncthreads();
+  uint32_t pre = inc - nt, tot = 0;
+  for (uint32_t q = 0; q < kMaxCoarse / 64; q++) {
+    pre += q < w ? ws[q] : 0u;
+    tot += ws[q];
+  }
+  if (c < Bc) {
+    t0[c] = pre;
+    for (uint32_t k = 0; k < nt; k++) tinfo[pre + k] = c;
+  }
+  if (c == 0) {
+    t0[Bc] = tot;
+    *ntb = tot;
+  }
+  for (uint32_t t = tot + c; t < ntmax; t += blockDim.x) tinfo[t] = kNone;
+}
 __device__ __forceinline__ void pb_tile(uint32_t t, const uint32_t *__restrict__ tinfo,
                                         const uint32_t *__restrict__ t0,
                                         const uint32_t *__restrict__ cstart, uint32_t &c,
@@ -592,6 +652,10 @@ __global__ __launch_bounds__(256) void k_pb_count(const uint32_t *__restrict__ t
                                                   const uint32_t *__restrict__ ikey,
                                                   uint32_t *__restrict__ hist) {
   __shared__ uint32_t h[256];
+  if (tinfo[blockIdx.x] == kNone) {  // past the last tile: zero its histogram slots
+    hist[256ull * blockIdx.x + threadIdx.x] = 0;
+    return;
+  }
   uint32_t c, i0, i1, nt, k;
   pb_tile(blockIdx.x, tinfo, t0, cstart, c, i0, i1, nt, k);
   h[threadIdx.x] = 0;
@@ -614,6 +678,7 @@ __global__ __launch_bounds__(kPartNT) void k_pb_scatter(const uint32_t *__restri
   __shared__ uint32_t skey[kSub], sbk[kSub];
   __shared__ float4 spos[kSub];
   __shared__ uint32_t ws[kPartNT / 64];
+  if (tinfo[blockIdx.x] == kNone) return;  // past the last tile
   uint32_t c, i0, i1, nt, k;
   pb_tile(blockIdx.x, tinfo, t0, cstart, c, i0, i1, nt, k);
   for (uint32_t b = threadIdx.x; b < 256; b += kPartNT) {
@@ -1227,7 +1292,7 @@ __global__ __launch_bounds__(256, 8) void k_link_sparse(uint32_t N, const float4
       }
     }
   }
-  add_counters(ctr, tests, rechecks);
+  add_counters(ctr, kCtrTestsSparse, tests, rechecks);
 }
 
 // ---------------------------------------------------------------------------
@@ -1388,7 +1453,7 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
   // one-wave cells: run sizes in LDS (home runs fit a byte: < 256 particles)
   constexpr bool SIZES = (NT == 64);
   __shared__ uint16_t bsz[SIZES ? MAXRUNSB : 1];
-  uint32_t tests = 0, rechecks = 0;
+  uint32_t tests = 0, rechecks = 0, joined = 0;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nd = *ndense;
   for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
@@ -1522,6 +1587,7 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
         uint32_t v = 2u;  // 2: nothing to do, 1: join, 0: test
         if (k < nq && gl == 0) {
           if (lfind(lp, sa) != lfind(lp, sb)) v = (uf_find(par, ha) == uf_find(par, hb)) ? 1u : 0u;
+          joined += (v != 0u) ? 1u : 0u;
         }
         v = __shfl(v, lane & ~15u, 64);
         const uint32_t sza = v == 0u ? run_end[ha] - ha : 0u;
@@ -1558,14 +1624,23 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
             const float4 a = pos4[ha], b = pos4[hb];
             tests++;
             if (!pair_linked(a, b, R, rechecks)) return;
+          } else {
+            joined++;
           }
           lunion(lp, sa, sb);
           return;
         }
       }
-      if (lfind(lp, sa) == lfind(lp, sb)) return;
-      if (ga == gb) lunion(lp, sa, sb);
-      else test_pair(sa, sb, ha, hb);
+      if (lfind(lp, sa) == lfind(lp, sb)) {
+        joined++;
+        return;
+      }
+      if (ga == gb) {
+        joined++;
+        lunion(lp, sa, sb);
+      } else {
+        test_pair(sa, sb, ha, hb);
+      }
     };
     // (A, A), in rounds each followed by a drain, so that merges found early
     // prune the later rounds (a full queue falls back to testing in place)
@@ -1726,20 +1801,31 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
     }
     __syncthreads();
   }
-  add_counters(ctr, tests, rechecks);
+  add_counters(ctr, NT == 64 ? kCtrTestsWave : kCtrTestsBig, tests, rechecks, joined);
 }
 
 // ---------------------------------------------------------------------------
 // 7: union-find finish
 // ---------------------------------------------------------------------------
+// Particle ID of input record p (the reference's P[p].ID, src_v2/vars.h:36-47).
+__device__ __forceinline__ unsigned long long rec_id(const uint8_t *__restrict__ base,
+                                                     uint32_t stride, uint32_t offid, uint32_t p) {
+  const uint32_t *q = reinterpret_cast<const uint32_t *>(base + (uint64_t)p * stride + offid);
+  return (unsigned long long)q[0] | ((unsigned long long)q[1] << 32);
+}
+
 // Flatten every particle to its root; the other members of a group
 // accumulate their minimum input index and their count at the root (minorig
-// and gsize start at ~0 and 0; the root itself is added in k_label).
+// and gsize start at ~0 and 0; the root itself is added in k_label).  With
+// idroot, the members' minimum particle ID is accumulated the same way.
 __global__ __launch_bounds__(256) void k_flatten(uint32_t N, uint32_t *__restrict__ par,
                                                  const float4 *__restrict__ pos4,
                                                  uint32_t *__restrict__ minorig,
                                                  uint32_t *__restrict__ gsize,
-                                                 unsigned long long *__restrict__ ctr) {
+                                                 unsigned long long *__restrict__ ctr,
+                                                 const uint8_t *__restrict__ base, uint32_t stride,
+                                                 uint32_t offid,
+                                                 unsigned long long *__restrict__ idroot) {
   uint32_t roots = 0;
   const uint32_t lane = threadIdx.x & 63;
   // wave-uniform loop: lanes of a wave reduce together
@@ -1774,13 +1860,22 @@ __global__ __launch_bounds__(256) void k_flatten(uint32_t N, uint32_t *__restric
         atomicMin(minorig + rl, v);
         atomicAdd(gsize + rl, (uint32_t)__popcll(m));
       }
+      if (idroot) {
+        unsigned long long id = mine ? rec_id(base, stride, offid, o) : ~0ull;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const unsigned long long t = __shfl_xor(id, off, 64);
+          id = t < id ? t : id;
+        }
+        if ((int)lane == leader) atomicMin(idroot + rl, id);
+      }
       pend &= ~m;
     }
   }
   uint64_t t = roots;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
-  if ((threadIdx.x & 63) == 0 && t) ctr_add(ctr, 2, t);
+  if ((threadIdx.x & 63) == 0 && t) ctr_add(ctr, kCtrGroups, t);
 }
 
 // Members of groups of two or more (always kept for the ring sort): the
@@ -1797,7 +1892,11 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
                                                uint32_t *__restrict__ pidx,
                                                uint32_t *__restrict__ pval,
                                                uint32_t *__restrict__ npairs,
-                                               uint32_t *__restrict__ hsize) {
+                                               uint32_t *__restrict__ hsize,
+                                               const uint8_t *__restrict__ base, uint32_t stride,
+                                               uint32_t offid,
+                                               const unsigned long long *__restrict__ idroot,
+                                               unsigned long long *__restrict__ idhead) {
   __shared__ uint32_t lk[kLabTile], lv[kLabTile];
   __shared__ uint32_t ln, lbase;
   const uint32_t lane = threadIdx.x & 63;
@@ -1817,7 +1916,13 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
         o = __float_as_uint(pos4[x].w);
         const uint32_t orr = (r == x) ? o : __float_as_uint(pos4[r].w);
         lab = min(minorig[r], orr);
-        if (o == lab) hsize[lab] = g + 1u;
+        if (o == lab) {
+          hsize[lab] = g + 1u;
+          if (idhead) {  // the root's own ID joins the members' minimum
+            const unsigned long long a = idroot[r], b = rec_id(base, stride, offid, orr);
+            idhead[lab] = a < b ? a : b;
+          }
+        }
       }
     }
     const unsigned long long b = __ballot(mem);
@@ -1862,19 +1967,28 @@ __global__ __launch_bounds__(256) void k_label_init(uint32_t N, const VolDev *__
   }
 }
 
-// Outputs for every particle: label_out (volume-local) and, for particles not
-// in the sorted list, the trivial ring entry next = self (1-based).
+// Outputs for every particle: label_out (volume-local), for particles not in
+// the sorted list the trivial ring entry next = self (1-based), and the
+// group's minimum particle ID (own ID for a singleton).
 __global__ __launch_bounds__(256) void k_out_init(uint32_t N, const uint32_t *__restrict__ label,
                                                   const uint32_t *__restrict__ vbeg, int nvol,
                                                   uint32_t *__restrict__ label_out,
-                                                  uint32_t *__restrict__ next_out) {
+                                                  uint32_t *__restrict__ next_out,
+                                                  const uint32_t *__restrict__ hsize,
+                                                  const uint8_t *__restrict__ base, uint32_t stride,
+                                                  uint32_t offid,
+                                                  const unsigned long long *__restrict__ idhead,
+                                                  unsigned long long *__restrict__ minid_out) {
   int hint = 0;
   const uint64_t i0 = (uint64_t)blockIdx.x * kTileN, i1 = min((uint64_t)N, i0 + kTileN);
   for (uint32_t i = (uint32_t)i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const uint32_t l = label[i];
     const uint32_t b = vbeg[vol_of_run(vbeg, nvol, i, N, hint)];
-    if (label_out) label_out[i] = (l & ~kFlag) - b;
+    const uint32_t head = l & ~kFlag;
+    if (label_out) label_out[i] = head - b;
     if (next_out && !(l & kFlag)) next_out[i] = i - b + 1u;
+    if (minid_out)
+      minid_out[i] = (head == i && hsize[i] == 1u) ? rec_id(base, stride, offid, i) : idhead[head];
   }
 }
 
@@ -1912,7 +2026,32 @@ __global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
 // ascending index order from the head (the order the host walks the ring
 // returned in next_out), so the host's Checkhalo / Output_Halos loops give
 // the same bits.  One wavefront per halo: the 64 lanes gather 64 members'
-// records at a time into LDS and lanes 0-5 add one component each, in order.
+// records at a time into LDS and then each of the first NQ lanes adds one
+// quantity over them, in order.  Quantities (q = lane):
+//   0-5    Pos x,y,z, Vel x,y,z                          (src_v2/FOF.c:805-810)
+//   with MOM, the second moments (src_v2/FOF.c:811-824):
+//   6-11   Pos xx,yy,zz,xy,xz,yz   12-17  Vel xx,...     raw products, ring order
+//   18-20  Pos x,y,z - head        21-23  Vel - head     deviations from the head
+//   24-29  Pos products of those   30-35  Vel products   (for centred moments)
+// A float difference widened to double is exact here (same-halo members), and
+// products of two such numbers (< 53 significant bits) are exact, so only the
+// additions round.
+__device__ __forceinline__ double halo_quantity(int q, const float (&v)[6], const double (&ref)[6]) {
+  if (q < 6) return (double)v[q];
+  // component pairs of the products: xx yy zz xy xz yz
+  const int r = q < 18 ? q - 6 : (q < 24 ? q - 18 : q - 24);
+  const int blk = r / 6, k = r % 6, o = blk * 3;  // blk 0: Pos, 1: Vel
+  if (q >= 18 && q < 24) {
+    const int c = r;  // 0-5: Pos xyz, Vel xyz
+    return (double)v[c] - ref[c];
+  }
+  const int ca = o + (k < 3 ? k : (k < 5 ? 0 : 1));
+  const int cb = o + (k < 3 ? k : (k == 3 ? 1 : 2));
+  if (q < 18) return (double)v[ca] * (double)v[cb];
+  return ((double)v[ca] - ref[ca]) * ((double)v[cb] - ref[cb]);
+}
+
+template <bool MOM>
 __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ dH,
                                                    const uint32_t *__restrict__ hk,
                                                    const uint32_t *__restrict__ skey,
@@ -1924,10 +2063,13 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
                                                    const VolDev *__restrict__ vols,
                                                    const uint32_t *__restrict__ vbeg, int nvol,
                                                    fof_halo_v1 *__restrict__ out,
+                                                   fof_moments_v1 *__restrict__ mout,
+                                                   uint64_t cap,
                                                    uint32_t *__restrict__ hhead,
                                                    unsigned long long *__restrict__ ctr) {
+  constexpr int NQ = MOM ? 36 : 6;
   // two stages per wave: the records of the next 64 members are loaded while
-  // lanes 0-5 add up the current ones
+  // the first NQ lanes add up the current ones
   __shared__ float stage[4][2][6][64];
   const uint32_t H = *dH;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1941,12 +2083,10 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
     hint = vol_from(vbeg, nvol, lab, hint);  // halos ascend by head, per wave too
     const int v = hint;
     const VolDev &V = vols[v];
-    // lanes 0-5 each accumulate one component in ring order (the host's
-    // order); the boundary test of src_v2/FOF.c:403-422 uses the staged
-    // positions of every member and is kept for owned halos only
     double acc = 0;
     uint32_t warn = 0;
     float r[6];
+    double ref[6];
     auto fetch = [&](uint32_t k) {
       if (lane < min(64u, n - k)) {
         const uint8_t *rec = base + (uint64_t)sval[k0 + k + lane] * stride;
@@ -1956,6 +2096,10 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
       }
     };
     fetch(0);
+    if (MOM) {  // the head (first member) is the reference point of the deviations
+#pragma unroll
+      for (int c = 0; c < 6; c++) ref[c] = (double)__shfl(r[c], 0, 64);
+    }
     int b = 0;
     for (uint32_t k = 0; k < n; k += 64, b ^= 1) {
       const uint32_t m = min(64u, n - k);
@@ -1972,8 +2116,18 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       if (k + 64 < n) fetch(k + 64);
-      if (lane < 6)
-        for (uint32_t q = 0; q < m; q++) acc += (double)stage[w][b][lane][q];
+      if (lane < NQ) {
+        if (lane < 6) {
+          for (uint32_t q = 0; q < m; q++) acc += (double)stage[w][b][lane][q];
+        } else {
+          for (uint32_t q = 0; q < m; q++) {
+            float vv[6];
+#pragma unroll
+            for (int c = 0; c < 6; c++) vv[c] = stage[w][b][c][q];
+            acc += halo_quantity(lane, vv, ref);
+          }
+        }
+      }
     }
     const double s0 = __shfl(acc, 0, 64), s1 = __shfl(acc, 1, 64), s2 = __shfl(acc, 2, 64);
     const double s3 = __shfl(acc, 3, 64), s4 = __shfl(acc, 4, 64), s5 = __shfl(acc, 5, 64);
@@ -1984,15 +2138,44 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
     const bool own = !((cx < V.rmin[0]) || (cx >= V.rmax[0]) || (cy < V.rmin[1]) ||
                        (cy >= V.rmax[1]) || (cz < V.rmin[2]) || (cz >= V.rmax[2]));
     if (!own) warn = 0;
+    if (MOM && mout && h < cap) {
+      // raw products (lanes 6-17), centred products from the deviations:
+      // sum (a - ma)(b - mb) = S_ab - S_a S_b / n with a, b taken about the head
+      double dev[6];
+#pragma unroll
+      for (int c = 0; c < 6; c++) dev[c] = __shfl(acc, 18 + c, 64);
+      double prod[12];
+#pragma unroll
+      for (int c = 0; c < 12; c++) prod[c] = __shfl(acc, 24 + c, 64);
+      const double rawq = __shfl(acc, 6 + lane % 12, 64);  // every lane takes part
+      if (lane < 12) {
+        const int blk = lane / 6, k = lane % 6, o = blk * 3;
+        const int ca = o + (k < 3 ? k : (k < 5 ? 0 : 1));
+        const int cb = o + (k < 3 ? k : (k == 3 ? 1 : 2));
+        double pc = 0, da = 0, db = 0;
+#pragma unroll
+        for (int c = 0; c < 12; c++) pc = (c == lane) ? prod[c] : pc;
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+          da = (c == ca) ? dev[c] : da;
+          db = (c == cb) ? dev[c] : db;
+        }
+        double *mo = reinterpret_cast<double *>(mout + h);
+        mo[lane] = rawq;                            // pos2[6], vel2[6]
+        mo[12 + lane] = pc - da * db / (double)n;   // cpos2[6], cvel2[6]
+      }
+    }
     if (lane == 0) {
-      fof_halo_v1 H1;
-      H1.head = lab - vbeg[v];
-      H1.n = n;
-      H1.owned = own ? 1u : 0u;
-      H1.edge_warn = warn;
-      H1.sum_pos[0] = s0, H1.sum_pos[1] = s1, H1.sum_pos[2] = s2;
-      H1.sum_vel[0] = s3, H1.sum_vel[1] = s4, H1.sum_vel[2] = s5;
-      out[h] = H1;
+      if (h < cap) {
+        fof_halo_v1 H1;
+        H1.head = lab - vbeg[v];
+        H1.n = n;
+        H1.owned = own ? 1u : 0u;
+        H1.edge_warn = warn;
+        H1.sum_pos[0] = s0, H1.sum_pos[1] = s1, H1.sum_pos[2] = s2;
+        H1.sum_vel[0] = s3, H1.sum_vel[1] = s4, H1.sum_vel[2] = s5;
+        out[h] = H1;
+      }
       hhead[h] = lab;
       owned_cnt += own ? 1u : 0u;
     }
@@ -2000,7 +2183,7 @@ __global__ __launch_bounds__(256) void k_halo_sums(const uint32_t *__restrict__ 
   uint64_t t = owned_cnt;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
-  if ((threadIdx.x & 63) == 0 && t) ctr_add(ctr, 3, t);
+  if ((threadIdx.x & 63) == 0 && t) ctr_add(ctr, kCtrOwned, t);
 }
 
 // halo_offsets[v] = first halo whose global head >= vbeg[v].
@@ -2168,277 +2351,403 @@ int prepare_volume(const fof_volume_v1 &in, double band, HostVol &hv, std::vecto
   return 0;
 }
 
-}  // namespace fof
-
-using namespace fof;
-
-extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1 *vols,
-                              uint32_t nvol, const fof_opts_v1 *opts, uint32_t *next_out,
-                              uint32_t *label_out, fof_halo_v1 *halos, uint64_t halo_cap,
-                              uint64_t *halo_offsets, uint64_t *nhalos_out,
-                              fof_stats_v1 *stats);
-
-extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1 *vols,
-                              uint32_t nvol, const fof_opts_v1 *opts, uint32_t *next_out,
-                              uint32_t *label_out, fof_halo_v1 *halos, uint64_t halo_cap,
-                              uint64_t *halo_offsets, uint64_t *nhalos_out,
-                              fof_stats_v1 *stats) {
-  EngineState &S = ctx->st;
-  Work &W = S.w;
-  hipStream_t st = S.stream;
-  const uint32_t flags = opts ? opts->flags : 0u;
-  double band = (opts && opts->band > 0) ? opts->band : 1e-5;
-  if (!(band >= 1e-6 && band <= 0.1)) return FOF_E_ARG;
-  const uint32_t T = std::min(kMaxT, (opts && opts->dense_threshold) ? opts->dense_threshold : kDefaultT);
-  const bool out_dev = (flags & FOF_OPT_OUT_DEVICE) != 0;
-  const bool want_ring = next_out && !(flags & FOF_OPT_NO_RING);
-  const bool want_halos = !(flags & FOF_OPT_NO_HALOS);
-  const bool timing = (flags & FOF_OPT_TIMING) != 0;
-
-  // ---- volumes ----
-  uint64_t N = 0, C = 0;
-  double Cd = 0;
-  int key_bits = 0;
-  std::vector<HostVol> hv(nvol);
-  std::vector<const fof_varlink_v1 *> vls;
-  std::vector<uint32_t> vbeg(nvol + 1);
-  for (uint32_t v = 0; v < nvol; v++) {
-    if (vols[v].offset != N) return FOF_E_ARG;  // volumes must be back to back
-    int rc = prepare_volume(vols[v], band, hv[v], vls);
-    if (rc) return rc;
-    vbeg[v] = (uint32_t)N;
-    hv[v].d.pbeg = (uint32_t)N;
-    hv[v].d.vid = v;
-    N += vols[v].n;
-    hv[v].d.pend = (uint32_t)N;
-    Cd += hv[v].ncell_d;
-    if (N >= 0x7FFFFFFFull) return FOF_E_RANGE;
+// Bump allocator over a workspace: run once with base == nullptr to size it,
+// then over the allocated arena to carve it.  Every buffer is 256-byte
+// aligned.
+struct Carve {
+  uint8_t *base = nullptr;
+  size_t off = 0;
+  template <class T>
+  T *take(size_t bytes) {
+    const size_t o = (off + 255) & ~(size_t)255;
+    off = o + bytes;
+    return base ? reinterpret_cast<T *>(base + o) : nullptr;
   }
-  vbeg[nvol] = (uint32_t)N;
-  if (N != P->n) return FOF_E_ARG;
-  // Dense cell tables unless they would dwarf the particle arrays.
-  const bool sparse = (flags & FOF_OPT_SPARSE) || Cd > 3.0 * (double)N + 4194304.0;
-  int volshift = 0;
-  if (sparse) {
+};
+
+// Host-side description of a call: volumes, grids, sizes of every buffer.
+struct Plan {
+  uint64_t N = 0, C = 0, Csz = 0;
+  double Cd = 0;
+  uint32_t nvol = 0;
+  bool sparse = false;
+  int key_bits = 0, volshift = 0;
+  uint32_t T = kDefaultT;
+  uint32_t big_grid = 8192;  // workgroups of the 256-thread dense kernels
+  double band = 1e-5;
+  std::vector<HostVol> hv;
+  std::vector<uint32_t> vbeg;
+  std::vector<double> vt_host;
+  std::vector<uint32_t> vt_off;  // per volume: first table entry (or ~0)
+  size_t vt_entries = 0;
+  bool varlink = false;
+  // dense grid partitions
+  uint32_t F = 0, Bc = 0, ntA = 0, ntBmax = 0;
+  uint64_t nA = 0, nB = 0;
+  uint64_t ndense_cap = 0, Hbound = 0, Hstage = 0;
+  size_t tmpb = 0;
+  // what the call produces
+  uint32_t flags = 0;
+  bool host_input = false, host_out = false;
+  bool want_ring = false, want_label = false, want_minid = false, want_halos = false,
+       want_mom = false;
+  bool stage_halos = false;  // halos (and moments) into the workspace first
+  uint32_t stride = 32, offpos = 8, offvel = 20;
+  int32_t offid = 0;
+};
+
+// Device buffers of a call, carved from the arena.
+struct Bufs {
+  float4 *pos4;
+  uint32_t *key, *start, *par, *runend, *rtab, *gsize, *label, *sortbuf, *dense, *hhead, *cnt,
+      *part, *vbeg, *small;
+  unsigned long long *fine, *idhead;
+  void *tmp;
+  VolDev *vols;
+  double *vtab;
+  uint64_t *hoff;
+  uint8_t *input;
+  uint32_t *st_next, *st_label;
+  unsigned long long *st_minid;
+  fof_halo_v1 *st_halo;
+  fof_moments_v1 *st_mom;
+};
+
+constexpr size_t kSmallBytes = 256 + (size_t)kCtrSlots * kCtrStride * 8;
+
+static size_t layout(const Plan &pl, uint8_t *base, Bufs &b) {
+  Carve c;
+  c.base = base;
+  const uint64_t N = pl.N, q4 = (N + 63) / 64 * 64;
+  b.pos4 = c.take<float4>(N * 16);
+  b.key = c.take<uint32_t>(N * 4 + 64);
+  b.start = c.take<uint32_t>((pl.Csz + 2) * 4 + 64);
+  b.par = c.take<uint32_t>(N * 4 + 64);
+  b.runend = c.take<uint32_t>(N * 4 + 64);
+  b.fine = c.take<unsigned long long>(N * 8 + 64);
+  b.rtab = c.take<uint32_t>(N * 4 + 64);
+  b.gsize = c.take<uint32_t>(N * 4 + 64);
+  b.label = c.take<uint32_t>(N * 4 + 64);
+  b.sortbuf = c.take<uint32_t>(16 * q4 + 256);
+  b.dense = c.take<uint32_t>(2 * pl.ndense_cap * 4 + 64);
+  b.tmp = c.take<void>(pl.tmpb);
+  b.vols = c.take<VolDev>(pl.nvol * sizeof(VolDev));
+  b.vtab = c.take<double>(pl.vt_entries * sizeof(double) + 8);
+  b.vbeg = c.take<uint32_t>((pl.nvol + 1) * 4 + 64);
+  b.small = c.take<uint32_t>(kSmallBytes);
+  b.hhead = c.take<uint32_t>(pl.Hbound * 4 + 64);
+  b.hoff = c.take<uint64_t>((pl.nvol + 1) * 8 + 64);
+  b.cnt = pl.sparse ? c.take<uint32_t>(N * 4 + 64) : nullptr;
+  b.part = pl.sparse ? nullptr
+                     : c.take<uint32_t>((2 * pl.nA + 2 * pl.nB + 2 * (uint64_t)pl.Bc + pl.ntBmax +
+                                         pl.F + 512) * 4);
+  b.input = pl.host_input ? c.take<uint8_t>(N * pl.stride) : nullptr;
+  b.idhead = pl.want_minid ? c.take<unsigned long long>(N * 8 + 64) : nullptr;
+  const bool ho = pl.host_out;
+  b.st_next = ho && pl.want_ring ? c.take<uint32_t>(N * 4) : nullptr;
+  b.st_label = ho && pl.want_label ? c.take<uint32_t>(N * 4) : nullptr;
+  b.st_minid = ho && pl.want_minid ? c.take<unsigned long long>(N * 8) : nullptr;
+  const bool hs = ho || pl.stage_halos;
+  b.st_halo = hs && pl.want_halos ? c.take<fof_halo_v1>(pl.Hstage * sizeof(fof_halo_v1)) : nullptr;
+  b.st_mom = hs && pl.want_mom ? c.take<fof_moments_v1>(pl.Hstage * sizeof(fof_moments_v1)) : nullptr;
+  return c.off + 256;
+}
+
+static int make_plan(const fof_particles_v1 *P, const fof_volume_v1 *vols, uint32_t nvol,
+                     const fof_opts_v1 *opts, const fof_outputs_v1 *out, Plan &pl) {
+  pl.flags = opts ? opts->flags : 0u;
+  pl.band = (opts && opts->band > 0) ? opts->band : 1e-5;
+  if (!(pl.band >= 1e-6 && pl.band <= 0.1)) return FOF_E_ARG;
+  pl.T = std::min(kMaxT, (opts && opts->dense_threshold) ? opts->dense_threshold : kDefaultT);
+  pl.big_grid = (opts && opts->big_grid) ? std::min(opts->big_grid, 65536u) : 8192u;
+  pl.nvol = nvol;
+  pl.stride = P->stride, pl.offpos = P->off_pos, pl.offvel = P->off_vel, pl.offid = P->off_id;
+  pl.host_input = P->location != FOF_MEM_DEVICE;
+  pl.hv.resize(nvol);
+  pl.vbeg.resize(nvol + 1);
+  pl.vt_off.assign(nvol, ~0u);
+  std::vector<const fof_varlink_v1 *> vls;
+  double nphmin = 1e300;
+  for (uint32_t v = 0; v < nvol; v++) {
+    if (vols[v].offset != pl.N) return FOF_E_ARG;  // volumes must be back to back
+    int rc = prepare_volume(vols[v], pl.band, pl.hv[v], vls);
+    if (rc) return rc;
+    pl.vbeg[v] = (uint32_t)pl.N;
+    pl.hv[v].d.pbeg = (uint32_t)pl.N;
+    pl.hv[v].d.vid = v;
+    pl.N += vols[v].n;
+    if (pl.N >= 0x7FFFFFFFull) return FOF_E_RANGE;
+    pl.hv[v].d.pend = (uint32_t)pl.N;
+    pl.Cd += pl.hv[v].ncell_d;
+    nphmin = std::min(nphmin, vols[v].domain.nphalomin);
+  }
+  pl.vbeg[nvol] = (uint32_t)pl.N;
+  if (pl.N != P->n) return FOF_E_ARG;
+  // Dense cell tables unless they would dwarf the particle arrays or reach
+  // the 2^31 cells a u32 cell index (with room for the table's end) allows.
+  pl.sparse = (pl.flags & FOF_OPT_SPARSE) || pl.Cd > 3.0 * (double)pl.N + 4194304.0 ||
+              pl.Cd >= (double)0x7FFFFFF0u;
+  if (pl.sparse) {
     int lb = 1, vb = 0;
     for (uint32_t v = 0; v < nvol; v++) {
       int b = 1;
-      while (b < 64 && (double)(1ull << b) < hv[v].ncell_d) b++;
+      while (b < 64 && (double)(1ull << b) < pl.hv[v].ncell_d) b++;
       lb = std::max(lb, b);
     }
     while ((1u << vb) < nvol) vb++;
-    if (lb + vb > 64 || hv[0].ncell_d > 1.8e19) return FOF_E_RANGE;
-    volshift = lb;
-    key_bits = lb + vb;
+    for (uint32_t v = 0; v < nvol; v++)
+      if (pl.hv[v].ncell_d > 1.8e19) return FOF_E_RANGE;
+    if (lb + vb > 64) return FOF_E_RANGE;
+    pl.volshift = lb;
+    pl.key_bits = lb + vb;
   } else {
     for (uint32_t v = 0; v < nvol; v++) {
-      hv[v].d.cbeg = (uint32_t)C;
-      hv[v].d.ncell = (uint32_t)hv[v].ncell;
-      C += hv[v].ncell;
-      if (C >= 0x7FFFFFF0ull) return FOF_E_RANGE;
+      pl.hv[v].d.cbeg = (uint32_t)pl.C;
+      pl.hv[v].d.ncell = (uint32_t)pl.hv[v].ncell;
+      pl.C += pl.hv[v].ncell;
     }
   }
-  const uint64_t Csz = sparse ? N : C;  // entries of the start table
-
-  // Variable linking-length tables are copied to the device back to back.
-  size_t vt_entries = 0;
+  pl.Csz = pl.sparse ? pl.N : pl.C;
   for (uint32_t v = 0; v < nvol; v++)
-    if (vols[v].domain.varlink) vt_entries += vols[v].domain.varlink->ntab;
-  std::vector<double> vt_host(vt_entries);
-  {
-    size_t o = 0;
-    for (uint32_t v = 0; v < nvol; v++) {
-      const fof_varlink_v1 *L = vols[v].domain.varlink;
-      if (!L) continue;
-      if (L->location == FOF_MEM_DEVICE)
-        FOF_HIP(hipMemcpy(&vt_host[o], L->l2tab, L->ntab * sizeof(double), hipMemcpyDeviceToHost));
-      else
-        memcpy(&vt_host[o], L->l2tab, L->ntab * sizeof(double));
-      for (uint32_t k = 0; k < L->ntab; k++)
-        if (!(vt_host[o + k] >= 0) || !std::isfinite(vt_host[o + k])) return FOF_E_ARG;
-      o += L->ntab;
+    if (vols[v].domain.varlink) {
+      pl.vt_off[v] = (uint32_t)pl.vt_entries;
+      pl.vt_entries += vols[v].domain.varlink->ntab;
     }
+  pl.varlink = pl.vt_entries > 0;
+  // outputs
+  const bool od = out->location == FOF_MEM_DEVICE;
+  pl.host_out = !od;
+  pl.want_ring = out->next && !(pl.flags & FOF_OPT_NO_RING);
+  pl.want_label = out->label != nullptr;
+  pl.want_minid = out->minid != nullptr;
+  if (pl.want_minid && P->off_id < 0) return FOF_E_ARG;
+  pl.want_halos = !(pl.flags & FOF_OPT_NO_HALOS);
+  pl.want_mom = pl.want_halos && out->moments != nullptr;
+  // sizes
+  const uint64_t N = pl.N;
+  pl.ndense_cap = N / (pl.T ? pl.T : 1) + 2;
+  // kept groups have at least max(1, ceil(nphalomin)) members and are disjoint
+  const double kmin = std::max(1.0, std::ceil(std::min(nphmin, 4e9)));
+  pl.Hbound = std::min<uint64_t>(N, (uint64_t)((double)N / kmin) + 1);
+  pl.Hstage = std::min<uint64_t>(pl.Hbound, out->halo_cap);
+  pl.tmpb = std::max(sort_tmp_bytes(N), compact_tmp_bytes(N));
+  if (!pl.sparse) {
+    pl.F = (uint32_t)((pl.C + kFineCells - 1) >> kFineBits);
+    pl.Bc = (pl.F + 255) >> 8;
+    if (pl.Bc > (uint32_t)kMaxCoarse) return FOF_E_RANGE;
+    pl.ntA = (uint32_t)((N + kPartTile - 1) / kPartTile);
+    pl.ntBmax = pl.ntA + pl.Bc;
+    pl.nA = (uint64_t)pl.Bc * pl.ntA;
+    pl.nB = 256ull * pl.ntBmax;
+    pl.tmpb = std::max(pl.tmpb, scan_tmp_bytes(std::max(pl.nA, pl.nB) + 1));
   }
+  // varlink tables (checked on the host)
+  pl.vt_host.resize(pl.vt_entries);
+  for (uint32_t v = 0; v < nvol; v++) {
+    const fof_varlink_v1 *L = vols[v].domain.varlink;
+    if (!L) continue;
+    double *dst = pl.vt_host.data() + pl.vt_off[v];
+    if (L->location == FOF_MEM_DEVICE)
+      FOF_HIP(hipMemcpy(dst, L->l2tab, L->ntab * sizeof(double), hipMemcpyDeviceToHost));
+    else
+      memcpy(dst, L->l2tab, L->ntab * sizeof(double));
+    for (uint32_t k = 0; k < L->ntab; k++)
+      if (!(dst[k] >= 0) || !std::isfinite(dst[k])) return FOF_E_ARG;
+  }
+  return 0;
+}
 
-  if (stats) {
-    memset(stats, 0, sizeof(*stats));
-    stats->version = FOF_ABI_VERSION;
-    stats->nvol = nvol;
-    stats->n = N;
-    stats->ncells = sparse ? 0 : C;
+// The volume table (VolDev[nvol], varlink tables, volume starts) as uploaded.
+static void build_vblob(const Plan &pl, const Bufs &b, std::vector<uint8_t> &blob) {
+  const size_t nv = pl.nvol * sizeof(VolDev);
+  blob.resize(nv + pl.vt_entries * sizeof(double) + (pl.nvol + 1) * 4);
+  for (uint32_t v = 0; v < pl.nvol; v++) {
+    VolDev V = pl.hv[v].d;
+    if (pl.vt_off[v] != ~0u) V.vtab = b.vtab + pl.vt_off[v];
+    memcpy(&blob[v * sizeof(VolDev)], &V, sizeof(VolDev));
   }
-  if (halo_offsets && !want_halos)
-    for (uint32_t v = 0; v <= nvol; v++) halo_offsets[v] = 0;
-  if (N == 0) {
-    if (nhalos_out) *nhalos_out = 0;
-    if (halo_offsets)
-      for (uint32_t v = 0; v <= nvol; v++) halo_offsets[v] = 0;
-    return 0;
-  }
+  if (pl.vt_entries) memcpy(&blob[nv], pl.vt_host.data(), pl.vt_entries * sizeof(double));
+  memcpy(&blob[nv + pl.vt_entries * sizeof(double)], pl.vbeg.data(), (pl.nvol + 1) * 4);
+}
 
-  // ---- workspace ----
-  const uint64_t ndense_cap = N / (T ? T : 1) + 2;
-  int rc = 0;
-  if ((rc = ensure(W.pos4, N * 16))) return rc;
-  if ((rc = ensure(W.key, N * 4 + 64))) return rc;
-  if ((rc = ensure(W.cnt, (sparse ? N : 1) * 4 + 64))) return rc;  // occupied-cell flags
-  if ((rc = ensure(W.start, (Csz + 2) * 4 + 64))) return rc;
-  if ((rc = ensure(W.par, N * 4 + 64))) return rc;
-  if ((rc = ensure(W.runend, N * 4 + 64))) return rc;
-  if ((rc = ensure(W.fine, N * 8 + 64))) return rc;
-  if ((rc = ensure(W.rtab, N * 4 + 64))) return rc;
-  if ((rc = ensure(W.gsize, N * 4 + 64))) return rc;
-  if ((rc = ensure(W.label, N * 4 + 64))) return rc;
-  const uint64_t q4 = (N + 63) / 64 * 64;  // u32 entries per sort array
-  if ((rc = ensure(W.sortbuf, 16 * q4 + 256))) return rc;
-  if ((rc = ensure(W.dense, 2 * ndense_cap * 4 + 64))) return rc;  // two lists
-  size_t tmpb = std::max(sort_tmp_bytes(N), compact_tmp_bytes(N));
-  {
-    // the partition scans: coarse buckets x tiles, 256 x tiles of the fine pass
-    const uint64_t F = (Csz + kFineCells - 1) >> kFineBits, Bc = (F + 255) >> 8;
-    const uint64_t ntA = (N + kPartTile - 1) / kPartTile;
-    tmpb = std::max(tmpb, scan_tmp_bytes(std::max(Bc * ntA, 256 * (ntA + Bc)) + 1));
-  }
-  if ((rc = ensure(W.tmp, tmpb))) return rc;
-  if ((rc = ensure(W.vols, nvol * sizeof(VolDev) + vt_entries * sizeof(double) + 256))) return rc;
-  if ((rc = ensure(W.vbeg, (nvol + 1) * 4 + 64))) return rc;
-  constexpr size_t kCtrBytes = (size_t)kCtrSlots * kCtrStride * 8;
-  if ((rc = ensure(W.small, 256 + kCtrBytes))) return rc;
-  if ((rc = ensure(W.hhead, N * 4 + 64))) return rc;
-  if ((rc = ensure(W.hoff, (nvol + 1) * 8 + 64))) return rc;
+static int upload_vblob(const Plan &pl, const Bufs &b, const std::vector<uint8_t> &blob,
+                        hipStream_t st) {
+  const size_t nv = pl.nvol * sizeof(VolDev), nt = pl.vt_entries * sizeof(double);
+  FOF_HIP(hipMemcpyAsync(b.vols, blob.data(), nv, hipMemcpyHostToDevice, st));
+  if (nt) FOF_HIP(hipMemcpyAsync(b.vtab, blob.data() + nv, nt, hipMemcpyHostToDevice, st));
+  FOF_HIP(hipMemcpyAsync(b.vbeg, blob.data() + nv + nt, (pl.nvol + 1) * 4, hipMemcpyHostToDevice,
+                         st));
+  return 0;
+}
 
-  // device copy of the particle array when it lives on the host
-  const uint8_t *dbase = static_cast<const uint8_t *>(P->base);
-  if (P->location != FOF_MEM_DEVICE) {
-    if ((rc = ensure(W.input, N * P->stride))) return rc;
-    FOF_HIP(hipMemcpyAsync(W.input.ptr, P->base, N * P->stride, hipMemcpyHostToDevice, st));
-    dbase = static_cast<const uint8_t *>(W.input.ptr);
-  }
-  uint32_t *d_next = next_out, *d_label = label_out;
-  if (!out_dev) {
-    if (want_ring) {
-      if ((rc = ensure(W.out_next, N * 4))) return rc;
-      d_next = static_cast<uint32_t *>(W.out_next.ptr);
-    }
-    if (label_out) {
-      if ((rc = ensure(W.out_label, N * 4))) return rc;
-      d_label = static_cast<uint32_t *>(W.out_label.ptr);
-    }
-  }
-  if (!want_ring) d_next = nullptr;
-
-  // volume table: VolDev[nvol] then the varlink tables
-  auto upload_vols = [&]() -> int {
-    std::vector<uint8_t> blob(nvol * sizeof(VolDev) + vt_entries * sizeof(double));
-    double *dtab = reinterpret_cast<double *>(static_cast<uint8_t *>(W.vols.ptr) + nvol * sizeof(VolDev));
-    size_t o = 0;
-    for (uint32_t v = 0; v < nvol; v++) {
-      VolDev V = hv[v].d;
-      if (vols[v].domain.varlink) {
-        V.vtab = dtab + o;
-        o += vols[v].domain.varlink->ntab;
-      }
-      memcpy(&blob[v * sizeof(VolDev)], &V, sizeof(VolDev));
-    }
-    if (vt_entries) memcpy(&blob[nvol * sizeof(VolDev)], vt_host.data(), vt_entries * sizeof(double));
-    FOF_HIP(hipMemcpyAsync(W.vols.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
-    FOF_HIP(hipMemcpyAsync(W.vbeg.ptr, vbeg.data(), (nvol + 1) * 4, hipMemcpyHostToDevice, st));
-    FOF_HIP(hipStreamSynchronize(st));  // blob / vbeg are stack-owned
-    return 0;
-  };
-  if ((rc = upload_vols())) return rc;
-  if (vt_entries && N) {
-    // A variable l: the sub-cell size follows the smallest l of any occupied
-    // cell (only occupied cells are home cells of a pair), not of the table.
-    unsigned long long *dmin = static_cast<unsigned long long *>(W.hoff.ptr);
-    FOF_HIP(hipMemsetAsync(dmin, 0xFF, nvol * 8, st));
-    k_varlink_min<<<(uint32_t)((N + 4 * kVlRun - 1) / (4 * kVlRun)), 256, 0, st>>>(dbase, P->stride, P->off_pos, (uint32_t)N,
-                                               static_cast<const VolDev *>(W.vols.ptr),
-                                               static_cast<const uint32_t *>(W.vbeg.ptr),
-                                               (int)nvol, dmin);
-    FOF_HIP(hipGetLastError());
-    std::vector<unsigned long long> hmin(nvol);
-    FOF_HIP(hipMemcpyAsync(hmin.data(), dmin, nvol * 8, hipMemcpyDeviceToHost, st));
-    FOF_HIP(hipStreamSynchronize(st));
-    for (uint32_t v = 0; v < nvol; v++) {
-      if (!vols[v].domain.varlink || hmin[v] == ~0ull) continue;
-      double l2;
-      memcpy(&l2, &hmin[v], 8);
-      set_subcells(hv[v].d, l2);
-    }
-    if ((rc = upload_vols())) return rc;
-  }
-
-  const VolDev *dV = static_cast<const VolDev *>(W.vols.ptr);
-  const uint32_t *dVB = static_cast<const uint32_t *>(W.vbeg.ptr);
-  float4 *pos4 = static_cast<float4 *>(W.pos4.ptr);
-  uint32_t *key = static_cast<uint32_t *>(W.key.ptr);
-  uint32_t *cnt = static_cast<uint32_t *>(W.cnt.ptr);
-  uint32_t *start = static_cast<uint32_t *>(W.start.ptr);
-  uint32_t *par = static_cast<uint32_t *>(W.par.ptr);
-  uint32_t *runend = static_cast<uint32_t *>(W.runend.ptr);
-  unsigned long long *fine = static_cast<unsigned long long *>(W.fine.ptr);
-  uint32_t *rtab = static_cast<uint32_t *>(W.rtab.ptr);
-  uint32_t *rtoff = key;  // free between the cell sort and k_label (head sizes)
-  uint32_t *gsize = static_cast<uint32_t *>(W.gsize.ptr);
-  uint32_t *label = static_cast<uint32_t *>(W.label.ptr);
-  // four u32 arrays; viewed as float4[N] they are the sub-cell sort buffer
-  uint32_t *sk0 = static_cast<uint32_t *>(W.sortbuf.ptr);
-  uint32_t *sv0 = sk0 + q4, *sk1 = sk0 + 2 * q4, *sv1 = sk0 + 3 * q4;
-  uint32_t *dense = static_cast<uint32_t *>(W.dense.ptr);
-  uint32_t *dbig = dense + ndense_cap;  // cells of kBigCell particles or more
-  // small: [0] error [1] small dense cells [2] M [3] H [4] overflow [5] U
-  //        [6] run-table fill [7] big dense cells [8] group members (label pairs)
-  uint32_t *small = static_cast<uint32_t *>(W.small.ptr);
-  unsigned long long *ctr = reinterpret_cast<unsigned long long *>(small + 64);  // slots
-  const uint32_t NN = (uint32_t)N;
-  const uint32_t stride = P->stride, offpos = P->off_pos, offvel = P->off_vel;
-  const uint32_t gN = grid_for(N);
-
-  if (timing && !S.events) {
-    for (auto &e : S.ev) FOF_HIP(hipEventCreate(&e));
-    S.events = true;
-  }
-  // Timing spans: events on the engine stream around each kernel group.
+// ---------------------------------------------------------------------------
+// One pipeline's share of a call: plan, carved workspace, output pointers.
+// ---------------------------------------------------------------------------
+struct Run {
+  Plan pl;
+  Bufs B;
+  Pipe *pp = nullptr;
+  const uint8_t *dbase = nullptr;  // particle records on the device
+  const void *host_src = nullptr;  // host records to upload first (host input)
+  uint32_t *d_next = nullptr, *d_label = nullptr;
+  unsigned long long *d_minid = nullptr;
+  fof_halo_v1 *d_halo = nullptr;
+  fof_moments_v1 *d_mom = nullptr;
+  uint64_t hcap = 0;
+  uint64_t poff = 0;    // first particle of this pipeline in the call
+  uint32_t v0 = 0;      // first volume of this pipeline in the call
+  bool timing = false;  // per-phase events on this pipeline's stream
   uint32_t spans = 0;
   int first_span = -1, last_span = -1;
+};
+
+// Plan a pipeline over volumes [v0, v0 + nv) of the call (rebased so that
+// the pipeline sees particles from 0) and carve its workspace.
+static int prepare_run(Run &r, Pipe &pp, const fof_particles_v1 *P, const fof_volume_v1 *vols,
+                       uint32_t v0, uint32_t nv, const fof_opts_v1 *opts,
+                       const fof_outputs_v1 *out, bool stage_halos) {
+  r.pp = &pp;
+  r.v0 = v0;
+  r.poff = vols[v0].offset;
+  fof_particles_v1 Pg = *P;
+  std::vector<fof_volume_v1> vg(vols + v0, vols + v0 + nv);
+  uint64_t n = 0;
+  for (auto &v : vg) {
+    v.offset -= r.poff;
+    n += v.n;
+  }
+  Pg.n = n;
+  Pg.base = static_cast<const uint8_t *>(P->base) + r.poff * P->stride;
+  int rc = make_plan(&Pg, vg.data(), nv, opts, out, r.pl);
+  if (rc) return rc;
+  Plan &pl = r.pl;
+  pl.stage_halos = stage_halos && pl.want_halos;
+  if (pl.N == 0) return 0;
+  const size_t need = layout(pl, nullptr, r.B);
+  // a workspace far larger than this call needs (a previous call with fewer
+  // pipelines) is given back rather than kept beside the others
+  if (pp.arena.bytes > 2 * need && pp.arena.bytes - need > (1ull << 30)) release(pp.arena);
+  if ((rc = ensure(pp.arena, need))) return rc;
+  layout(pl, static_cast<uint8_t *>(pp.arena.ptr), r.B);
+  if (pl.host_input) {
+    r.host_src = Pg.base;
+    r.dbase = r.B.input;
+  } else {
+    r.dbase = static_cast<const uint8_t *>(Pg.base);
+  }
+  const bool od = !pl.host_out;
+  r.d_next = pl.want_ring ? (od ? out->next + r.poff : r.B.st_next) : nullptr;
+  r.d_label = pl.want_label ? (od ? out->label + r.poff : r.B.st_label) : nullptr;
+  r.d_minid = pl.want_minid ? (od ? reinterpret_cast<unsigned long long *>(out->minid) + r.poff
+                                  : r.B.st_minid)
+                            : nullptr;
+  const bool direct = od && !pl.stage_halos;
+  r.d_halo = pl.want_halos ? (direct ? out->halos : r.B.st_halo) : nullptr;
+  r.d_mom = pl.want_mom ? (direct ? out->moments : r.B.st_mom) : nullptr;
+  r.hcap = r.d_halo ? (direct ? out->halo_cap : pl.Hstage) : 0;
+  build_vblob(pl, r.B, pp.vblob);
+  pp.hres.resize(kSmallBytes + (nv + 1) * 8);
+  return 0;
+}
+
+// Uploads that precede a pipeline's kernels: the volume table and, for
+// host-resident input, the particle records.
+static int upload_run(Run &r, hipStream_t st) {
+  if (r.pl.N == 0) return 0;
+  if (r.host_src)
+    FOF_HIP(hipMemcpyAsync(r.B.input, r.host_src, r.pl.N * r.pl.stride, hipMemcpyHostToDevice, st));
+  return upload_vblob(r.pl, r.B, r.pp->vblob, st);
+}
+
+// Every kernel of a pipeline, in stream order.  Without a variable linking
+// length or occupied-cell addressing nothing here waits for the host, so the
+// sequence can be captured in a graph.
+static int enqueue_run(Run &r, hipStream_t st) {
+  Plan &pl = r.pl;
+  Bufs &B = r.B;
+  Pipe &pp = *r.pp;
+  const uint64_t N = pl.N;
+  if (N == 0) return 0;
+  const uint32_t nvol = pl.nvol;
+  int rc = 0;
+  const uint8_t *dbase = r.dbase;
+  uint32_t *small = B.small;
+  unsigned long long *ctr = reinterpret_cast<unsigned long long *>(small + kSmCtr);
+  uint32_t *hsm = reinterpret_cast<uint32_t *>(pp.hres.data());
+  const bool timing = r.timing;
+  if (timing && !pp.events) {
+    for (auto &e : pp.ev) FOF_HIP(hipEventCreate(&e));
+    pp.events = true;
+  }
+  r.spans = 0;
+  r.first_span = r.last_span = -1;
   auto tb = [&](int k) -> int {
     if (timing) {
-      FOF_HIP(hipEventRecord(S.ev[2 * k], st));
-      if (first_span < 0) first_span = k;
+      FOF_HIP(hipEventRecord(pp.ev[2 * k], st));
+      if (r.first_span < 0) r.first_span = k;
     }
     return 0;
   };
   auto te = [&](int k) -> int {
     if (timing) {
-      FOF_HIP(hipEventRecord(S.ev[2 * k + 1], st));
-      spans |= 1u << k;
-      last_span = k;
+      FOF_HIP(hipEventRecord(pp.ev[2 * k + 1], st));
+      r.spans |= 1u << k;
+      r.last_span = k;
     }
     return 0;
   };
+  if ((rc = fill_u32(small, kSmallBytes / 4, 0u, st))) return rc;
 
-  FOF_HIP(hipMemsetAsync(small, 0, 256 + kCtrBytes, st));
+  if (pl.varlink) {
+    // A variable l: the sub-cell size follows the smallest l of any occupied
+    // cell (only occupied cells are home cells of a pair), not of the table.
+    unsigned long long *dmin = reinterpret_cast<unsigned long long *>(B.hoff);
+    if ((rc = fill_u32(reinterpret_cast<uint32_t *>(dmin), 2ull * nvol, ~0u, st))) return rc;
+    k_varlink_min<<<(uint32_t)((N + 4 * kVlRun - 1) / (4 * kVlRun)), 256, 0, st>>>(
+        dbase, pl.stride, pl.offpos, (uint32_t)N, B.vols, B.vbeg, (int)nvol, dmin);
+    FOF_HIP(hipGetLastError());
+    std::vector<unsigned long long> hmin(nvol);
+    FOF_HIP(hipMemcpyAsync(hmin.data(), dmin, nvol * 8, hipMemcpyDeviceToHost, st));
+    FOF_HIP(hipStreamSynchronize(st));
+    for (uint32_t v = 0; v < nvol; v++) {
+      if (pl.vt_off[v] == ~0u || hmin[v] == ~0ull) continue;
+      double l2;
+      memcpy(&l2, &hmin[v], 8);
+      set_subcells(pl.hv[v].d, l2);
+    }
+    build_vblob(pl, B, pp.vblob);
+    if ((rc = upload_vblob(pl, B, pp.vblob, st))) return rc;
+  }
+
+  const VolDev *dV = B.vols;
+  const uint32_t *dVB = B.vbeg;
+  float4 *pos4 = B.pos4;
+  uint32_t *key = B.key, *start = B.start, *par = B.par, *runend = B.runend, *rtab = B.rtab;
+  unsigned long long *fine = B.fine;
+  uint32_t *rtoff = key;  // free between the cell sort and k_label (head sizes)
+  uint32_t *gsize = B.gsize, *label = B.label;
+  const uint64_t q4 = (N + 63) / 64 * 64;
+  // four u32 arrays; viewed as float4[N] they are the sub-cell sort buffer
+  uint32_t *sk0 = B.sortbuf, *sv0 = sk0 + q4, *sk1 = sk0 + 2 * q4, *sv1 = sk0 + 3 * q4;
+  uint32_t *dense = B.dense, *dbig = dense + pl.ndense_cap;
+  const uint32_t NN = (uint32_t)N;
+  const uint32_t stride = pl.stride, offpos = pl.offpos, offvel = pl.offvel;
+  const uint32_t offid = pl.offid >= 0 ? (uint32_t)pl.offid : 0u;
+  const uint32_t gN = grid_for(N);
+  const uint32_t T = pl.T;
+
   Grid G;
   G.start = start;
   G.htab = nullptr;
   G.cell_of = label;  // sparse: the label buffer is free until k_label
   G.hmask = 0;
-  G.sparse = sparse ? 1 : 0;
-  G.volshift = volshift;
-  uint32_t hsmall[16];
-  uint32_t ndense = 0, nbig = 0;  // dense cells below / from kBigCell particles
-  if (!sparse) {
+  G.sparse = pl.sparse ? 1 : 0;
+  G.volshift = pl.volshift;
+  if (!pl.sparse) {
     // 1-3: counting sort by cell: partitions A (coarse) and B (fine), then C
-    const uint32_t F = (uint32_t)((C + kFineCells - 1) >> kFineBits);  // fine buckets
-    const uint32_t Bc = (F + 255) >> 8;                                 // coarse buckets
-    if (Bc > (uint32_t)kMaxCoarse) return FOF_E_RANGE;
-    const uint32_t ntA = (uint32_t)((N + kPartTile - 1) / kPartTile);
-    const uint32_t ntBmax = ntA + Bc;
-    const uint64_t nA = (uint64_t)Bc * ntA, nB = 256ull * ntBmax;
-    if ((rc = ensure(W.part, (2 * nA + 2 * nB + 2 * (uint64_t)Bc + ntBmax + F + 512) * 4)))
-      return rc;
-    uint32_t *histA = static_cast<uint32_t *>(W.part.ptr);
+    const uint32_t F = pl.F, Bc = pl.Bc, ntA = pl.ntA, ntBmax = pl.ntBmax;
+    const uint64_t nA = pl.nA, nB = pl.nB;
+    uint32_t *histA = B.part;
     uint32_t *offA = histA + nA + 32;      // nA + 1
     uint32_t *histB = offA + nA + 32;      // nB
     uint32_t *offB = histB + nB + 32;      // nB + 1
@@ -2451,125 +2760,100 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
     float4 *p2 = reinterpret_cast<float4 *>(sk0);
     if ((rc = tb(0))) return rc;
     k_pa_count<<<ntA, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, Bc, ntA, key,
-                                    histA, small + 0);
+                                    histA, small + kSmErr);
     FOF_HIP(hipGetLastError());
     if ((rc = te(0))) return rc;
     if ((rc = tb(1))) return rc;
-    if ((rc = scan_u32(histA, offA, nA, W.tmp.ptr, st))) return rc;
+    if ((rc = scan_u32(histA, offA, nA, B.tmp, st))) return rc;
     k_pa_starts<<<grid_for(Bc + 1), 256, 0, st>>>(Bc, ntA, offA, cst);
     FOF_HIP(hipGetLastError());
-    std::vector<uint32_t> hc(Bc + 1), ht0(Bc + 1);
-    FOF_HIP(hipMemcpyAsync(hc.data(), cst, (Bc + 1) * 4, hipMemcpyDeviceToHost, st));
-    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
-    FOF_HIP(hipStreamSynchronize(st));
-    if (hsmall[0]) return FOF_E_DOMAIN;
-    std::vector<uint32_t> hti;
-    hti.reserve(ntBmax);
-    for (uint32_t c = 0; c < Bc; c++) {
-      ht0[c] = (uint32_t)hti.size();
-      const uint32_t nt = (hc[c + 1] - hc[c] + kPartTile - 1) / kPartTile;
-      for (uint32_t k = 0; k < nt; k++) hti.push_back(c);
-    }
-    ht0[Bc] = (uint32_t)hti.size();
-    const uint32_t ntB = ht0[Bc];
-    FOF_HIP(hipMemcpyAsync(t0, ht0.data(), (Bc + 1) * 4, hipMemcpyHostToDevice, st));
-    if (ntB) FOF_HIP(hipMemcpyAsync(tinfo, hti.data(), ntB * 4, hipMemcpyHostToDevice, st));
-    FOF_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
+    k_pb_tiles<<<1, kMaxCoarse, 0, st>>>(Bc, ntBmax, cst, t0, tinfo, small + kSmNtB);
+    FOF_HIP(hipGetLastError());
     if ((rc = te(1))) return rc;
     if ((rc = tb(2))) return rc;
     k_pa_scatter<<<ntA, kPartNT, 0, st>>>(dbase, stride, offpos, NN, Bc, ntA, key, offA, k1, p1);
     FOF_HIP(hipGetLastError());
-    if (ntB) {
-      k_pb_count<<<ntB, 256, 0, st>>>(tinfo, t0, cst, k1, histB);
-      FOF_HIP(hipGetLastError());
-      if ((rc = scan_u32(histB, offB, 256ull * ntB, W.tmp.ptr, st))) return rc;
-      k_pb_scatter<<<ntB, kPartNT, 0, st>>>(tinfo, t0, cst, offB, k1, p1, k2, p2);
-      FOF_HIP(hipGetLastError());
-    }
+    k_pb_count<<<ntBmax, 256, 0, st>>>(tinfo, t0, cst, k1, histB);
+    FOF_HIP(hipGetLastError());
+    if ((rc = scan_u32(histB, offB, nB, B.tmp, st))) return rc;
+    k_pb_scatter<<<ntBmax, kPartNT, 0, st>>>(tinfo, t0, cst, offB, k1, p1, k2, p2);
+    FOF_HIP(hipGetLastError());
     k_pb_starts<<<grid_for(F + 1), 256, 0, st>>>(F, t0, cst, offB, NN, fst);
     FOF_HIP(hipGetLastError());
-    k_pc_sort<<<F, kPcThreads, 0, st>>>((uint64_t)C, fst, k2, p2, start, pos4, dV, (int)nvol, T, dense,
-                                 small + 1, dbig, small + 7);
+    k_pc_sort<<<F, kPcThreads, 0, st>>>((uint64_t)pl.C, fst, k2, p2, start, pos4, dV, (int)nvol, T,
+                                        dense, small + kSmNDense, dbig, small + kSmNBig);
     FOF_HIP(hipGetLastError());
     if ((rc = te(2))) return rc;
-    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
-    FOF_HIP(hipStreamSynchronize(st));
-    ndense = hsmall[1];
-    nbig = hsmall[7];
   } else {
     // 1-3: composite keys, LSD sort by (volume, cell), occupied cells, hash
+    uint32_t *cnt = B.cnt;
     if ((rc = tb(0))) return rc;
     k_keys_sparse<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, dV, dVB, (int)nvol, G, sk0, key,
-                                      sv0, small + 0);
+                                      sv0, small + kSmErr);
     FOF_HIP(hipGetLastError());
     if ((rc = te(0))) return rc;
-    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
-    FOF_HIP(hipStreamSynchronize(st));
-    if (hsmall[0]) return FOF_E_DOMAIN;
     if ((rc = tb(1))) return rc;
-    if ((rc = radix_sort_u32(sk0, sv0, sk1, sv1, N, std::min(key_bits, 32), W.tmp.ptr, st))) return rc;
-    if (key_bits > 32) {
+    if ((rc = radix_sort_u32(sk0, sv0, sk1, sv1, N, nullptr, std::min(pl.key_bits, 32), B.tmp, st)))
+      return rc;
+    if (pl.key_bits > 32) {
       k_gather_u32<<<gN, 256, 0, st>>>(NN, key, sv0, gsize);
       FOF_HIP(hipGetLastError());
-      if ((rc = radix_sort_u32(gsize, sv0, sk1, sv1, N, key_bits - 32, W.tmp.ptr, st))) return rc;
+      if ((rc = radix_sort_u32(gsize, sv0, sk1, sv1, N, nullptr, pl.key_bits - 32, B.tmp, st)))
+        return rc;
     }
     if ((rc = te(1))) return rc;
     if ((rc = tb(2))) return rc;
-    k_scatter_sparse<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, sv0, dV, dVB, (int)nvol, G, pos4,
-                                         par, runend);
+    k_scatter_sparse<<<gN, 256, 0, st>>>(dbase, stride, offpos, NN, sv0, dV, dVB, (int)nvol, G,
+                                         pos4, par, runend);
     FOF_HIP(hipGetLastError());
     if ((rc = te(2))) return rc;
     k_cell_flags<<<gN, 256, 0, st>>>(NN, par, runend, cnt);
     FOF_HIP(hipGetLastError());
-    if ((rc = compact_flagged(cnt, N, kFlag, nullptr, start, small + 5, W.tmp.ptr, st))) return rc;
-    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
+    if ((rc = compact_flagged(cnt, N, nullptr, kFlag, nullptr, start, small + kSmU, B.tmp, st)))
+      return rc;
+    // the hash table is sized from the occupied-cell count
+    FOF_HIP(hipMemcpyAsync(hsm, small, 64, hipMemcpyDeviceToHost, st));
     FOF_HIP(hipStreamSynchronize(st));
-    const uint32_t U = hsmall[5];
+    const uint32_t U = hsm[kSmU];
     uint64_t hsz = 1024;
     while (hsz < 2ull * U) hsz <<= 1;
-    if ((rc = ensure(W.hash, hsz * sizeof(HEntry) + 256))) return rc;
-    HEntry *htab = static_cast<HEntry *>(W.hash.ptr);
-    FOF_HIP(hipMemsetAsync(htab, 0xFF, hsz * sizeof(HEntry), st));
+    if ((rc = ensure(pp.hash, hsz * sizeof(HEntry) + 256))) return rc;
+    HEntry *htab = static_cast<HEntry *>(pp.hash.ptr);
+    if ((rc = fill_u32(reinterpret_cast<uint32_t *>(htab), hsz * 4, ~0u, st))) return rc;
     G.htab = htab;
     G.hmask = hsz - 1;
-    k_cells_finish<<<grid_for(U), 256, 0, st>>>(small + 5, NN, start, par, runend, G, htab, label,
-                                                dV, dVB, (int)nvol, T, dense, small + 1, dbig,
-                                                small + 7);
+    k_cells_finish<<<grid_for(U), 256, 0, st>>>(small + kSmU, NN, start, par, runend, G, htab,
+                                                label, dV, dVB, (int)nvol, T, dense,
+                                                small + kSmNDense, dbig, small + kSmNBig);
     FOF_HIP(hipGetLastError());
-    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
-    FOF_HIP(hipStreamSynchronize(st));
-    ndense = hsmall[1];
-    nbig = hsmall[7];
-    if (stats) stats->ncells = U;
   }
   if ((rc = tb(3))) return rc;
   k_init_uf<<<gN, 256, 0, st>>>(NN, par, runend);
   FOF_HIP(hipGetLastError());
   if ((rc = te(3))) return rc;
-  // 4: sub-cell sort of dense cells (tmp4 aliases the sort buffers s0..s3)
-  if (ndense + nbig) {
+  // 4: sub-cell sort of dense cells (tmp4 aliases the sort buffers s0..s3);
+  // grids for the most dense cells there can be, the lists' lengths are read
+  // on the device
+  const uint32_t gsml = (uint32_t)std::min<uint64_t>(pl.ndense_cap, 65536);
+  const uint32_t gbig = (uint32_t)std::min<uint64_t>(N / kBigCell + 1, pl.big_grid);
+  bool any_sub = false;
+  for (uint32_t v = 0; v < nvol; v++) any_sub = any_sub || pl.hv[v].d.m[0] > 0;
+  if (any_sub) {
     if ((rc = tb(4))) return rc;
-    const uint32_t lists[2] = {ndense, nbig};
-    uint32_t *lptr[2] = {dense, dbig};
-    if (lists[0]) {  // cells below kBigCell particles: one wave each
-      k_subsort<64, true><<<std::min(lists[0], 65536u), 64, 0, st>>>(
-          lptr[0], small + 1, start, dV, dVB, (int)nvol, pos4, reinterpret_cast<float4 *>(sk0),
-          par, runend, fine, rtoff, rtab);
-      FOF_HIP(hipGetLastError());
-    }
-    if (lists[1]) {
-      k_subsort<256, false><<<std::min(lists[1], 8192u), 256, 0, st>>>(
-          lptr[1], small + 7, start, dV, dVB, (int)nvol, pos4, reinterpret_cast<float4 *>(sk0),
-          par, runend, fine, rtoff, rtab);
-      FOF_HIP(hipGetLastError());
-    }
+    k_subsort<64, true><<<gsml, 64, 0, st>>>(dense, small + kSmNDense, start, dV, dVB, (int)nvol,
+                                             pos4, reinterpret_cast<float4 *>(sk0), par, runend,
+                                             fine, rtoff, rtab);
+    FOF_HIP(hipGetLastError());
+    k_subsort<256, false><<<gbig, 256, 0, st>>>(dbig, small + kSmNBig, start, dV, dVB, (int)nvol,
+                                                pos4, reinterpret_cast<float4 *>(sk0), par, runend,
+                                                fine, rtoff, rtab);
+    FOF_HIP(hipGetLastError());
     if ((rc = te(4))) return rc;
   }
   // 5-6: linking
   if ((rc = tb(5))) return rc;
-  bool fast = !sparse;
-  for (uint32_t v = 0; v < nvol; v++)
-    fast = fast && hv[v].d.vntab == 0 && hv[v].d.m[0] > 0;
+  bool fast = !pl.sparse;
+  for (uint32_t v = 0; v < nvol; v++) fast = fast && pl.hv[v].d.vntab == 0 && pl.hv[v].d.m[0] > 0;
   if (fast)
     k_link_sparse<true><<<gN, 256, 0, st>>>(NN, pos4, G, runend, fine, par, dV, dVB, (int)nvol, T, ctr);
   else
@@ -2579,170 +2863,589 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   // the most sub-cells (so runs) a dense cell of the batch can have
   uint32_t maxbins = 1;
   for (uint32_t v = 0; v < nvol; v++)
-    if (hv[v].d.m[0] > 0)
-      maxbins = std::max(maxbins, (uint32_t)(hv[v].d.m[0] * hv[v].d.m[1] * hv[v].d.m[2]));
-  if (nbig) {
+    if (pl.hv[v].d.m[0] > 0)
+      maxbins = std::max(maxbins, (uint32_t)(pl.hv[v].d.m[0] * pl.hv[v].d.m[1] * pl.hv[v].d.m[2]));
+  if (any_sub) {
     if ((rc = tb(6))) return rc;
     constexpr int kRB = 448;
     const uint32_t ra = std::min(maxbins, (uint32_t)kMaxRuns);
     k_link_dense<256, 8, kMaxRuns, kRB, 512, 1024, 1>
-        <<<std::min(nbig, 8192u), 256, dense_lds(ra, kRB, false), st>>>(
-            dbig, small + 7, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol,
-            ctr, small + 4, ra);
+        <<<gbig, 256, dense_lds(ra, kRB, false), st>>>(
+            dbig, small + kSmNBig, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol,
+            ctr, small + kSmOverflow, ra);
     FOF_HIP(hipGetLastError());
     if ((rc = te(6))) return rc;
-  }
-  if (ndense) {
     if ((rc = tb(15))) return rc;
-    constexpr int kRB = 64;
-    const uint32_t ra = std::min(maxbins, kBigCell - 1);
-    k_link_dense<64, 6, (int)kBigCell - 1, kRB, 64, 256, 8>
-        <<<std::min(ndense, 65536u), 64, dense_lds(ra, kRB, true), st>>>(
-            dense, small + 1, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol,
-            ctr, small + 4, ra);
+    constexpr int kRBw = 64;
+    const uint32_t raw = std::min(maxbins, kBigCell - 1);
+    k_link_dense<64, 6, (int)kBigCell - 1, kRBw, 64, 256, 8>
+        <<<gsml, 64, dense_lds(raw, kRBw, true), st>>>(
+            dense, small + kSmNDense, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB,
+            (int)nvol, ctr, small + kSmOverflow, raw);
     FOF_HIP(hipGetLastError());
     if ((rc = te(15))) return rc;
   }
-  // 7: union-find finish (minorig reuses runend, hsize reuses key)
+  // 7: union-find finish (minorig reuses runend, hsize reuses key, the
+  // members' minimum ID reuses fine)
   uint32_t *minorig = runend, *hsize = key;
+  unsigned long long *idroot = pl.want_minid ? fine : nullptr;
   if ((rc = tb(7))) return rc;
-  FOF_HIP(hipMemsetAsync(minorig, 0xFF, N * 4, st));
-  FOF_HIP(hipMemsetAsync(gsize, 0, N * 4, st));
-  k_flatten<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, ctr);
+  if ((rc = fill_u32(minorig, N, ~0u, st))) return rc;
+  if ((rc = fill_u32(gsize, N, 0u, st))) return rc;
+  if (idroot && (rc = fill_u32(reinterpret_cast<uint32_t *>(idroot), 2 * N, ~0u, st))) return rc;
+  k_flatten<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, ctr, dbase, stride, offid, idroot);
   FOF_HIP(hipGetLastError());
   if ((rc = te(7))) return rc;
   if ((rc = tb(8))) return rc;
   k_label_init<<<tiles_for(N), 256, 0, st>>>(NN, dV, dVB, (int)nvol, label, hsize);
   FOF_HIP(hipGetLastError());
-  {
-    const uint64_t nt = (N + kLabTile - 1) / kLabTile;
-    k_label<<<(uint32_t)nt, 256, 0, st>>>(NN, par, pos4, minorig, gsize, sk0, sv0, small + 8,
-                                          hsize);
-    FOF_HIP(hipGetLastError());
-    FOF_HIP(hipMemcpyAsync(hsmall, small, 16 * 4, hipMemcpyDeviceToHost, st));
-    FOF_HIP(hipStreamSynchronize(st));
-    if ((rc = scatter_u32(sk0, sv0, sk1, sv1, hsmall[8], N, label, W.tmp.ptr, st))) return rc;
-  }
-  k_out_init<<<tiles_for(N), 256, 0, st>>>(NN, label, dVB, (int)nvol, d_label, d_next);
+  k_label<<<(uint32_t)((N + kLabTile - 1) / kLabTile), 256, 0, st>>>(
+      NN, par, pos4, minorig, gsize, sk0, sv0, small + kSmPairs, hsize, dbase, stride, offid,
+      idroot, B.idhead);
+  FOF_HIP(hipGetLastError());
+  if ((rc = scatter_u32(sk0, sv0, sk1, sv1, N, small + kSmPairs, N, label, B.tmp, st))) return rc;
+  k_out_init<<<tiles_for(N), 256, 0, st>>>(NN, label, dVB, (int)nvol, r.d_label, r.d_next, hsize,
+                                           dbase, stride, offid, B.idhead, r.d_minid);
   FOF_HIP(hipGetLastError());
   if ((rc = te(8))) return rc;
   // 8: kept groups sorted by label (stable: ascending index inside a group)
   if ((rc = tb(9))) return rc;
-  if ((rc = compact_flagged(label, N, kFlag, sk0, sv0, small + 2, W.tmp.ptr, st))) return rc;
+  if ((rc = compact_flagged(label, N, nullptr, kFlag, sk0, sv0, small + kSmM, B.tmp, st))) return rc;
   if ((rc = te(9))) return rc;
-  FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
-  FOF_HIP(hipStreamSynchronize(st));
-  const uint32_t M = hsmall[2];
-  if (hsmall[4]) return FOF_E_RANGE;  // run table overflow (cannot happen with m <= 12)
   if ((rc = tb(10))) return rc;
-  if ((rc = radix_sort_u32(sk0, sv0, sk1, sv1, M, bits_for(N), W.tmp.ptr, st))) return rc;
+  if ((rc = radix_sort_u32(sk0, sv0, sk1, sv1, N, small + kSmM, bits_for(N), B.tmp, st))) return rc;
   if ((rc = te(10))) return rc;
   // 9: ring + halos
-  uint64_t H = 0;
-  if (M) {
-    if ((rc = tb(11))) return rc;
+  if ((rc = tb(11))) return rc;
+  {
     // ring pairs in the (now free) cell-sorted position buffer
     uint32_t *p32 = reinterpret_cast<uint32_t *>(pos4);
-    uint32_t *pidx = d_next ? p32 : nullptr;
-    k_ring<<<tiles_for(M), 256, 0, st>>>(small + 2, sk0, sv0, hsize, dV, dVB, (int)nvol, pidx,
+    uint32_t *pidx = r.d_next ? p32 : nullptr;
+    k_ring<<<tiles_for(N), 256, 0, st>>>(small + kSmM, sk0, sv0, hsize, dV, dVB, (int)nvol, pidx,
                                         p32 + N, sk1);
     FOF_HIP(hipGetLastError());
-    if (d_next &&
-        (rc = scatter_u32(p32, p32 + N, p32 + 2 * N, p32 + 3 * N, M, N, d_next, W.tmp.ptr, st)))
+    if (r.d_next && (rc = scatter_u32(p32, p32 + N, p32 + 2 * N, p32 + 3 * N, N, small + kSmM, N,
+                                      r.d_next, B.tmp, st)))
       return rc;
-    if ((rc = te(11))) return rc;
   }
-  if (want_halos && M) {
+  if ((rc = te(11))) return rc;
+  if (pl.want_halos) {
     // halo starts: positions k of the heads of groups passing the size cut
     if ((rc = tb(12))) return rc;
-    if ((rc = compact_flagged(sk1, M, kFlag, nullptr, sv1, small + 3, W.tmp.ptr, st))) return rc;
+    if ((rc = compact_flagged(sk1, N, small + kSmM, kFlag, nullptr, sv1, small + kSmH, B.tmp, st)))
+      return rc;
     if ((rc = te(12))) return rc;
-    FOF_HIP(hipMemcpyAsync(hsmall, small, 8 * 4, hipMemcpyDeviceToHost, st));
-    FOF_HIP(hipStreamSynchronize(st));
-    H = hsmall[3];
-    if (nhalos_out) *nhalos_out = H;
-    if (halos && H > halo_cap) return FOF_E_RANGE;
-    if (H) {
-      fof_halo_v1 *hout = halos;
-      if (!out_dev || !halos) {
-        if ((rc = ensure(W.out_halo, H * sizeof(fof_halo_v1)))) return rc;
-        hout = static_cast<fof_halo_v1 *>(W.out_halo.ptr);
-      }
-      if ((rc = tb(13))) return rc;
-      k_halo_sums<<<grid_for(H * 64), 256, 0, st>>>(small + 3, sv1, sk0, sv0, hsize, dbase, stride,
-                                               offpos, offvel, dV, dVB, (int)nvol, hout,
-                                               static_cast<uint32_t *>(W.hhead.ptr), ctr);
-      FOF_HIP(hipGetLastError());
-      if ((rc = te(13))) return rc;
-      if (halos && !out_dev)
-        FOF_HIP(hipMemcpyAsync(halos, hout, H * sizeof(fof_halo_v1), hipMemcpyDeviceToHost, st));
+    if ((rc = tb(13))) return rc;
+    const uint32_t gh = (uint32_t)std::min<uint64_t>((pl.Hbound * 64 + 255) / 256, 8192);
+    if (r.d_mom)
+      k_halo_sums<true><<<gh, 256, 0, st>>>(small + kSmH, sv1, sk0, sv0, hsize, dbase, stride,
+                                            offpos, offvel, dV, dVB, (int)nvol, r.d_halo, r.d_mom,
+                                            r.hcap, B.hhead, ctr);
+    else
+      k_halo_sums<false><<<gh, 256, 0, st>>>(small + kSmH, sv1, sk0, sv0, hsize, dbase, stride,
+                                             offpos, offvel, dV, dVB, (int)nvol, r.d_halo, nullptr,
+                                             r.hcap, B.hhead, ctr);
+    FOF_HIP(hipGetLastError());
+    if ((rc = te(13))) return rc;
+    k_halo_offsets<<<grid_for(nvol + 1), 256, 0, st>>>(small + kSmH, B.hhead, dVB, (int)nvol, B.hoff);
+    FOF_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+// Counts, flags, counters and halo offsets back to the host (after the
+// pipeline's kernels, on the same stream).
+static int readback_run(Run &r, hipStream_t st) {
+  if (r.pl.N == 0) return 0;
+  FOF_HIP(hipMemcpyAsync(r.pp->hres.data(), r.B.small, kSmallBytes, hipMemcpyDeviceToHost, st));
+  if (r.pl.want_halos)
+    FOF_HIP(hipMemcpyAsync(r.pp->hres.data() + kSmallBytes, r.B.hoff, (r.pl.nvol + 1) * 8,
+                           hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+static uint32_t run_nhalos(const Run &r) {
+  if (r.pl.N == 0 || !r.pl.want_halos) return 0;
+  return reinterpret_cast<const uint32_t *>(r.pp->hres.data())[kSmH];
+}
+
+// Staged halos (and moments) of the pipelines of a call into the output, at
+// each pipeline's prefix of the halo counts (read on the device).
+struct HaloParts {
+  const fof_halo_v1 *src[kMaxPipes];
+  const fof_moments_v1 *msrc[kMaxPipes];
+  const uint32_t *cnt[kMaxPipes];
+  uint64_t stage[kMaxPipes];
+  int k;
+};
+
+__global__ __launch_bounds__(256) void k_gather_halos(HaloParts hp, fof_halo_v1 *__restrict__ out,
+                                                      fof_moments_v1 *__restrict__ mout,
+                                                      uint64_t cap) {
+  uint64_t base = 0;
+  for (int g = 0; g < hp.k; g++) {
+    const uint64_t h = hp.cnt[g] ? min((uint64_t)*hp.cnt[g], hp.stage[g]) : 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < h; i += (uint64_t)gridDim.x * 256) {
+      if (base + i >= cap) break;
+      if (out) out[base + i] = hp.src[g][i];
+      if (mout) mout[base + i] = hp.msrc[g][i];
     }
-    if (halo_offsets) {
-      uint64_t *doff = static_cast<uint64_t *>(W.hoff.ptr);
-      k_halo_offsets<<<grid_for(nvol + 1), 256, 0, st>>>(
-          small + 3, static_cast<uint32_t *>(W.hhead.ptr), dVB, (int)nvol, doff);
+    base += hp.cnt[g] ? *hp.cnt[g] : 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Autotuning: dense-cell threshold, concurrent pipelines and the grid of the
+// 256-thread dense kernels, per workload shape.
+// ---------------------------------------------------------------------------
+static uint32_t ilog2(uint64_t x) {
+  uint32_t b = 0;
+  while (x > 1) x >>= 1, b++;
+  return b;
+}
+
+// Shape of a call: log2 of particles and volumes, addressing, variable l,
+// particles per cell (x16, log2) and the linking length over the cell size
+// (x64).  Calls with one key behave alike for the tuned parameters.
+static TuneKey tune_key(const Plan &pl) {
+  TuneKey k;
+  k.lg_n = ilog2(pl.N);
+  k.lg_nvol = ilog2(pl.nvol);
+  k.sparse = pl.sparse ? 1 : 0;
+  k.varlink = pl.varlink ? 1 : 0;
+  k.ppc16 = ilog2((uint64_t)(16.0 * (double)pl.N / std::max(1.0, pl.Cd)) + 1);
+  double b = 0;
+  for (const HostVol &h : pl.hv) b += h.d.linklength / h.d.lcell[0];
+  k.b64 = (uint32_t)std::lround(64.0 * b / std::max<size_t>(1, pl.hv.size()));
+  return k;
+}
+
+static std::mutex g_tune_mu;
+
+static const char *tune_path() {
+  const char *p = getenv("FOF_TUNE_CACHE");
+  return (p && *p) ? p : nullptr;
+}
+
+static bool tune_lookup(EngineState &S, const TuneKey &k, TuneVal &v) {
+  for (auto &e : S.tuned)
+    if (e.first == k) {
+      v = e.second;
+      return true;
+    }
+  const char *path = tune_path();
+  if (!path) return false;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  FILE *f = fopen(path, "r");
+  if (!f) return false;
+  char line[256];
+  bool found = false;
+  while (fgets(line, sizeof line, f)) {
+    TuneKey q;
+    TuneVal w;
+    if (sscanf(line, "%u %u %u %u %u %u -> %u %u %u", &q.lg_n, &q.lg_nvol, &q.sparse, &q.varlink,
+               &q.ppc16, &q.b64, &w.T, &w.pipes, &w.gbig) == 9 &&
+        q == k) {
+      v = w;  // the last entry for a key wins
+      found = true;
+    }
+  }
+  fclose(f);
+  if (found) S.tuned.push_back({k, v});
+  return found;
+}
+
+static void tune_store(EngineState &S, const TuneKey &k, const TuneVal &v) {
+  S.tuned.push_back({k, v});
+  const char *path = tune_path();
+  if (!path) return;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  FILE *f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%u %u %u %u %u %u -> %u %u %u\n", k.lg_n, k.lg_nvol, k.sparse, k.varlink, k.ppc16,
+          k.b64, v.T, v.pipes, v.gbig);
+  fclose(f);
+}
+
+}  // namespace fof
+
+using namespace fof;
+
+extern "C" int fof_engine_plan(const fof_particles_v1 *P, const fof_volume_v1 *vols,
+                               uint32_t nvol, const fof_opts_v1 *opts, fof_plan_v1 *out) {
+  Plan pl;
+  fof_outputs_v1 o;
+  memset(&o, 0, sizeof(o));
+  o.location = FOF_MEM_DEVICE;
+  o.label = reinterpret_cast<uint32_t *>(1);  // a typical call: ring, labels, halos
+  o.next = reinterpret_cast<uint32_t *>(1);
+  int rc = make_plan(P, vols, nvol, opts, &o, pl);
+  if (rc) return rc;
+  Bufs b;
+  memset(out, 0, sizeof(*out));
+  out->version = FOF_ABI_VERSION;
+  out->sparse = pl.sparse ? 1u : 0u;
+  out->n = pl.N;
+  out->cells = pl.sparse ? 0 : pl.C;
+  out->grid_cells = pl.Cd;
+  out->workspace_bytes = layout(pl, nullptr, b);
+  out->dense_threshold = pl.T;
+  return 0;
+}
+
+namespace fof {
+
+// FNV-1a over the bytes that fix a call's launch sequence.
+struct Sig {
+  uint64_t h = 1469598103934665603ull;
+  void add(const void *p, size_t n) {
+    const uint8_t *b = static_cast<const uint8_t *>(p);
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+  }
+  template <class T>
+  void add(const T &x) {
+    add(&x, sizeof(T));
+  }
+};
+
+// Volumes split into k contiguous groups of about equal particle counts.
+static std::vector<uint32_t> split_volumes(const fof_volume_v1 *vols, uint32_t nvol, uint32_t k) {
+  uint64_t tot = 0;
+  for (uint32_t v = 0; v < nvol; v++) tot += vols[v].n;
+  std::vector<uint32_t> cut{0};
+  uint64_t acc = 0;
+  for (uint32_t v = 0; v < nvol; v++) {
+    acc += vols[v].n;
+    const uint32_t g = (uint32_t)cut.size();
+    if (g < k && acc * k >= tot * g && v + 1 < nvol && nvol - (v + 1) >= k - g) cut.push_back(v + 1);
+  }
+  cut.push_back(nvol);
+  return cut;
+}
+
+// One call with a fixed configuration: pipelines `pipes`, graph replay when
+// asked for and possible.
+static int run_call(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1 *vols,
+                    uint32_t nvol, const fof_opts_v1 *opts, fof_outputs_v1 *out, uint32_t pipes,
+                    double *wall_ms) {
+  EngineState &S = ctx->st;
+  const uint32_t flags = opts ? opts->flags : 0u;
+  fof_stats_v1 *stats = out->stats;
+  out->nhalos = 0;
+  if (out->halo_offsets)
+    for (uint32_t v = 0; v <= nvol; v++) out->halo_offsets[v] = 0;
+  pipes = std::max(1u, std::min({pipes, nvol, (uint32_t)kMaxPipes}));
+  const std::vector<uint32_t> cut = split_volumes(vols, nvol, pipes);
+  const uint32_t K = (uint32_t)cut.size() - 1;
+  if (S.pipes.empty()) S.pipes.push_back(nullptr);  // pipeline 0 is S.main
+  while (S.pipes.size() < K) {
+    S.pipes.push_back(new Pipe);
+    FOF_HIP(hipStreamCreate(&S.pipes.back()->stream));
+  }
+  if (!S.sync_events) {
+    FOF_HIP(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
+    for (auto &e : S.join) FOF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    FOF_HIP(hipEventCreate(&S.t0));
+    FOF_HIP(hipEventCreate(&S.t1));
+    S.sync_events = true;
+  }
+  std::vector<Run> runs(K);
+  int rc = 0;
+  const bool timing = (flags & FOF_OPT_TIMING) != 0;
+  for (uint32_t g = 0; g < K; g++) {
+    Pipe &pp = g == 0 ? S.main : *S.pipes[g];
+    if ((rc = prepare_run(runs[g], pp, P, vols, cut[g], cut[g + 1] - cut[g], opts, out, K > 1)))
+      return rc;
+  }
+  const Plan &pl0 = runs[0].pl;
+  uint64_t Ntot = 0;
+  bool varlink = false, sparse = false, host_io = false;
+  for (auto &r : runs) {
+    Ntot += r.pl.N;
+    varlink = varlink || r.pl.varlink;
+    sparse = sparse || r.pl.sparse;
+    host_io = host_io || r.pl.host_input || r.pl.host_out;
+  }
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->version = FOF_ABI_VERSION;
+    stats->nvol = nvol;
+    stats->n = Ntot;
+    stats->dense_threshold = pl0.T;
+    stats->big_grid = pl0.big_grid;
+    stats->sparse = sparse ? 1u : 0u;
+    stats->pipelines = K;
+    for (auto &r : runs) stats->ncells += r.pl.sparse ? 0 : r.pl.C;
+  }
+  if (Ntot == 0) return 0;
+  hipStream_t st = S.main.stream;
+  const bool want_halos = pl0.want_halos;
+  const bool gather = K > 1 && want_halos && !pl0.host_out && (out->halos || out->moments);
+  const bool graph = (flags & FOF_OPT_GRAPH) && !varlink && !sparse && !host_io;
+  // the pipelines' kernels, forked from and joined back to the main stream
+  auto enqueue_all = [&](bool with_timing) -> int {
+    for (uint32_t g = 0; g < K; g++) runs[g].timing = with_timing && g == 0;
+    if (K > 1) FOF_HIP(hipEventRecord(S.fork, st));
+    for (uint32_t g = 0; g < K; g++) {
+      hipStream_t sg = runs[g].pp->stream;
+      if (g > 0) FOF_HIP(hipStreamWaitEvent(sg, S.fork, 0));
+      int e = enqueue_run(runs[g], sg);
+      if (e) return e;
+      if (g > 0) {
+        FOF_HIP(hipEventRecord(S.join[g], sg));
+        FOF_HIP(hipStreamWaitEvent(st, S.join[g], 0));
+      }
+    }
+    if (gather) {
+      HaloParts hp;
+      memset(&hp, 0, sizeof(hp));
+      hp.k = (int)K;
+      for (uint32_t g = 0; g < K; g++) {
+        hp.src[g] = runs[g].B.st_halo;
+        hp.msrc[g] = runs[g].B.st_mom;
+        hp.cnt[g] = runs[g].pl.N ? runs[g].B.small + kSmH : nullptr;
+        hp.stage[g] = runs[g].pl.Hstage;
+      }
+      k_gather_halos<<<1024, 256, 0, st>>>(hp, out->halos, pl0.want_mom ? out->moments : nullptr,
+                                           out->halo_cap);
       FOF_HIP(hipGetLastError());
-      FOF_HIP(hipMemcpyAsync(halo_offsets, doff, (nvol + 1) * 8, hipMemcpyDeviceToHost, st));
+    }
+    return 0;
+  };
+  // uploads, on the main stream ahead of everything
+  for (auto &r : runs)
+    if ((rc = upload_run(r, st))) return rc;
+  FOF_HIP(hipEventRecord(S.t0, st));
+  uint32_t graph_state = 0;
+  if (graph) {
+    Sig sig;
+    sig.add(*P);
+    sig.add(nvol);
+    for (uint32_t v = 0; v < nvol; v++) sig.add(vols[v]);
+    if (opts) sig.add(*opts);
+    sig.add(out->location);
+    sig.add(out->next), sig.add(out->label), sig.add(out->minid), sig.add(out->halos);
+    sig.add(out->moments), sig.add(out->halo_cap);
+    sig.add(K);
+    for (auto &r : runs) sig.add(r.pp->arena.ptr);
+    if (S.graph_exec && S.graph_sig == sig.h) {
+      FOF_HIP(hipGraphLaunch(S.graph_exec, st));
+      graph_state = 1;
+    } else {
+      if (S.graph_exec) (void)hipGraphExecDestroy(S.graph_exec);
+      S.graph_exec = nullptr;
+      FOF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_all(false);
+      hipGraph_t gr = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &gr);
+      if (rc) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+      }
+      if (e != hipSuccess) return fof_set_hip_error(e, "hipStreamEndCapture", __FILE__, __LINE__);
+      e = hipGraphInstantiate(&S.graph_exec, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (e != hipSuccess) {
+        S.graph_exec = nullptr;
+        return fof_set_hip_error(e, "hipGraphInstantiate", __FILE__, __LINE__);
+      }
+      S.graph_sig = sig.h;
+      FOF_HIP(hipGraphLaunch(S.graph_exec, st));
+      graph_state = 2;
     }
   } else {
-    if (nhalos_out) *nhalos_out = 0;
-    if (halo_offsets)
-      for (uint32_t v = 0; v <= nvol; v++) halo_offsets[v] = 0;
+    if ((rc = enqueue_all(timing))) return rc;
   }
-  if ((rc = tb(14))) return rc;
-  if (!out_dev) {
-    if (want_ring) FOF_HIP(hipMemcpyAsync(next_out, d_next, N * 4, hipMemcpyDeviceToHost, st));
-    if (label_out) FOF_HIP(hipMemcpyAsync(label_out, d_label, N * 4, hipMemcpyDeviceToHost, st));
-  }
-  if ((rc = te(14))) return rc;
-  unsigned long long hcs[kCtrSlots * kCtrStride], hc[4] = {0, 0, 0, 0};
-  FOF_HIP(hipMemcpyAsync(hcs, ctr, sizeof(hcs), hipMemcpyDeviceToHost, st));
+  FOF_HIP(hipEventRecord(S.t1, st));
+  for (auto &r : runs)
+    if ((rc = readback_run(r, st))) return rc;
   FOF_HIP(hipStreamSynchronize(st));
-  for (int q = 0; q < kCtrSlots; q++)
-    for (int k = 0; k < 4; k++) hc[k] += hcs[q * kCtrStride + k];
+  float tot_ms = 0;
+  FOF_HIP(hipEventElapsedTime(&tot_ms, S.t0, S.t1));
+  if (wall_ms) *wall_ms = tot_ms;
+
+  // ---- results ----
+  uint64_t H = 0;
+  bool dom_err = false, ovf = false;
+  for (auto &r : runs) {
+    if (r.pl.N == 0) continue;
+    const uint32_t *hsm = reinterpret_cast<const uint32_t *>(r.pp->hres.data());
+    dom_err = dom_err || hsm[kSmErr];
+    ovf = ovf || hsm[kSmOverflow];
+    H += run_nhalos(r);
+  }
+  out->nhalos = H;
+  if (dom_err) return FOF_E_DOMAIN;
+  if (ovf) return FOF_E_RANGE;  // run table overflow (cannot happen with m <= 12)
+  if (want_halos && H > out->halo_cap && (out->halos || out->moments)) {
+    if (out->halo_offsets) out->halo_offsets[nvol] = H;  // the capacity needed
+    if (stats) stats->nhalos = H;
+    return FOF_E_RANGE;
+  }
+  if (out->halo_offsets && want_halos) {
+    uint64_t base = 0;
+    for (auto &r : runs) {
+      const uint64_t *ho = reinterpret_cast<const uint64_t *>(r.pp->hres.data() + kSmallBytes);
+      for (uint32_t v = 0; v < r.pl.nvol; v++) out->halo_offsets[r.v0 + v] = base + (r.pl.N ? ho[v] : 0);
+      base += run_nhalos(r);
+    }
+    out->halo_offsets[nvol] = H;
+  }
+  // host outputs
+  if (pl0.host_out) {
+    uint64_t hb = 0;
+    for (auto &r : runs) {
+      const uint64_t n = r.pl.N, h = run_nhalos(r);
+      if (n) {
+        if (r.pl.want_ring)
+          FOF_HIP(hipMemcpyAsync(out->next + r.poff, r.B.st_next, n * 4, hipMemcpyDeviceToHost, st));
+        if (r.pl.want_label)
+          FOF_HIP(hipMemcpyAsync(out->label + r.poff, r.B.st_label, n * 4, hipMemcpyDeviceToHost, st));
+        if (r.pl.want_minid)
+          FOF_HIP(hipMemcpyAsync(out->minid + r.poff, r.B.st_minid, n * 8, hipMemcpyDeviceToHost, st));
+        if (out->halos && h)
+          FOF_HIP(hipMemcpyAsync(out->halos + hb, r.B.st_halo, h * sizeof(fof_halo_v1),
+                                 hipMemcpyDeviceToHost, st));
+        if (out->moments && h)
+          FOF_HIP(hipMemcpyAsync(out->moments + hb, r.B.st_mom, h * sizeof(fof_moments_v1),
+                                 hipMemcpyDeviceToHost, st));
+      }
+      hb += h;
+    }
+    FOF_HIP(hipStreamSynchronize(st));
+  }
   if (stats) {
-    stats->ndense = ndense + nbig;
-    stats->pair_tests = hc[0];
-    stats->rechecks = hc[1];
-    stats->ngroups = hc[2];
-    stats->nmembers = M;
+    unsigned long long hc[kCtrStride] = {};
+    for (auto &r : runs) {
+      if (r.pl.N == 0) continue;
+      const uint32_t *hsm = reinterpret_cast<const uint32_t *>(r.pp->hres.data());
+      const unsigned long long *hcs = reinterpret_cast<const unsigned long long *>(hsm + kSmCtr);
+      for (int q = 0; q < kCtrSlots; q++)
+        for (int k = 0; k < kCtrStride; k++) hc[k] += hcs[q * kCtrStride + k];
+      stats->ndense += (uint64_t)hsm[kSmNDense] + hsm[kSmNBig];
+      stats->nmembers += hsm[kSmM];
+      if (r.pl.sparse) stats->ncells += hsm[kSmU];
+    }
+    stats->pair_tests_kernel[0] = hc[kCtrTestsSparse];
+    stats->pair_tests_kernel[1] = hc[kCtrTestsBig];
+    stats->pair_tests_kernel[2] = hc[kCtrTestsWave];
+    stats->pair_tests = hc[kCtrTestsSparse] + hc[kCtrTestsBig] + hc[kCtrTestsWave];
+    stats->pairs_joined = hc[kCtrJoined];
+    stats->rechecks = hc[kCtrRechecks];
+    stats->ngroups = hc[kCtrGroups];
     stats->nhalos = H;
-    stats->nhalos_owned = hc[3];
-    stats->sparse = sparse ? 1u : 0u;
-    stats->dense_threshold = T;
+    stats->nhalos_owned = hc[kCtrOwned];
+    stats->graph = graph_state;
     if (timing) {
+      const Run &r = runs[0];
+      Pipe &pp = *r.pp;
       float ms;
-      for (int k = 0; k < FOF_NPHASE - 1; k++) {
-        if (!(spans & (1u << k))) continue;
-        FOF_HIP(hipEventElapsedTime(&ms, S.ev[2 * k], S.ev[2 * k + 1]));
-        stats->phase_ms[k] = ms;
-      }
-      if (first_span >= 0) {
-        FOF_HIP(hipEventElapsedTime(&ms, S.ev[2 * first_span], S.ev[2 * last_span + 1]));
-        stats->phase_ms[FOF_NPHASE - 1] = ms;
-      }
+      if (!graph_state)
+        for (int k = 0; k < FOF_NPHASE - 1; k++) {
+          if (!(r.spans & (1u << k))) continue;
+          FOF_HIP(hipEventElapsedTime(&ms, pp.ev[2 * k], pp.ev[2 * k + 1]));
+          stats->phase_ms[k] = ms;
+        }
+      stats->phase_ms[FOF_NPHASE - 1] = tot_ms;
       stats->link_kernel_ms = stats->phase_ms[5] + stats->phase_ms[6] + stats->phase_ms[15];
     }
   }
   return 0;
 }
 
+}  // namespace fof
+
+extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1 *vols,
+                              uint32_t nvol, const fof_opts_v1 *opts, fof_outputs_v1 *out) {
+  EngineState &S = ctx->st;
+  fof_opts_v1 o;
+  memset(&o, 0, sizeof(o));
+  o.version = FOF_ABI_VERSION;
+  if (opts) o = *opts;
+  uint32_t pipes = o.pipelines ? o.pipelines : 1u;
+  if (!(o.flags & FOF_OPT_AUTOTUNE)) return run_call(ctx, P, vols, nvol, &o, out, pipes, nullptr);
+  // Autotuned: the shape's cached choice, or a search on this call's own
+  // input (every candidate links the same groups, so the last run's outputs
+  // are the call's outputs).
+  Plan pl;
+  fof_outputs_v1 probe = *out;
+  int rc = make_plan(P, vols, nvol, &o, &probe, pl);
+  if (rc) return rc;
+  const TuneKey key = tune_key(pl);
+  TuneVal best;
+  if (!tune_lookup(S, key, best)) {
+    fof_stats_v1 *user_stats = out->stats;
+    out->stats = nullptr;
+    auto trial = [&](uint32_t T, uint32_t K, uint32_t gb, double &ms) -> int {
+      fof_opts_v1 q = o;
+      q.flags &= ~(FOF_OPT_TIMING | FOF_OPT_GRAPH);
+      q.dense_threshold = T;
+      q.big_grid = gb;
+      double best_ms = 1e300;
+      for (int rep = 0; rep < 2; rep++) {  // the first run also warms the workspace
+        double w = 0;
+        int e = run_call(ctx, P, vols, nvol, &q, out, K, &w);
+        if (e) return e;
+        best_ms = std::min(best_ms, w);
+      }
+      ms = best_ms;
+      return 0;
+    };
+    best = TuneVal{kDefaultT, 1, 0};
+    double bms = 1e300, ms;
+    for (uint32_t T : {16u, 24u, 32u}) {
+      if ((rc = trial(T, 1, 0, ms))) break;
+      if (ms < bms) bms = ms, best.T = T;
+    }
+    for (uint32_t gb : {2048u, 32768u}) {
+      if (rc || (rc = trial(best.T, 1, gb, ms))) break;
+      if (ms < bms * 0.99) bms = ms, best.gbig = gb;
+    }
+    for (uint32_t K : {2u, 4u}) {
+      if (rc || K > nvol) break;
+      if ((rc = trial(best.T, K, best.gbig, ms))) break;
+      if (ms < bms * 0.99) bms = ms, best.pipes = K;
+    }
+    out->stats = user_stats;
+    if (rc) return rc;
+    tune_store(S, key, best);
+  }
+  o.dense_threshold = best.T;
+  o.big_grid = best.gbig;
+  return run_call(ctx, P, vols, nvol, &o, out, best.pipes, nullptr);
+}
+
 extern "C" int fof_engine_init(fof_ctx *ctx) {
-  FOF_HIP(hipStreamCreate(&ctx->st.stream));
+  FOF_HIP(hipStreamCreate(&ctx->st.main.stream));
   return 0;
+}
+
+static void free_pipe(Pipe &pp) {
+  release(pp.arena);
+  release(pp.hash);
+  if (pp.events)
+    for (auto &e : pp.ev) (void)hipEventDestroy(e);
+  pp.events = false;
+  if (pp.stream) (void)hipStreamDestroy(pp.stream);
+  pp.stream = nullptr;
 }
 
 extern "C" void fof_engine_free(fof_ctx *ctx) {
   EngineState &S = ctx->st;
-  Scratch *all[] = {&S.w.pos4, &S.w.key, &S.w.cnt, &S.w.start, &S.w.par, &S.w.runend,
-                    &S.w.gsize, &S.w.label, &S.w.sortbuf,
-                    &S.w.dense, &S.w.tmp, &S.w.vols, &S.w.vbeg, &S.w.small, &S.w.input,
-                    &S.w.out_next, &S.w.out_label, &S.w.out_halo, &S.w.hhead, &S.w.hoff, &S.w.part,
-                    &S.w.hash, &S.w.fine, &S.w.rtab};
-  for (Scratch *s : all) release(*s);
-  if (S.events)
-    for (auto &e : S.ev) (void)hipEventDestroy(e);
-  S.events = false;
-  if (S.stream) (void)hipStreamDestroy(S.stream);
-  S.stream = nullptr;
+  if (S.graph_exec) (void)hipGraphExecDestroy(S.graph_exec);
+  S.graph_exec = nullptr;
+  free_pipe(S.main);
+  for (auto &p : S.pipes) {
+    if (!p) continue;
+    free_pipe(*p);
+    delete p;
+  }
+  S.pipes.clear();
+  if (S.sync_events) {
+    (void)hipEventDestroy(S.fork);
+    for (auto &e : S.join) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(S.t0);
+    (void)hipEventDestroy(S.t1);
+  }
+  S.sync_events = false;
 }

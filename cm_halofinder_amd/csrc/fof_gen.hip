@@ -531,7 +531,7 @@ extern "C" int fof_ghosts_device(const fof_ghost_v1 *g, const void *in, uint64_t
   while ((1 << bits) < nvol) bits++;
   if (nvol > 1) {
     if ((rc = ensure(tmp, sort_tmp_bytes(m)))) return rc;
-    if ((rc = radix_sort_u32(vk, it, vk1, it1, m, bits, tmp.ptr, 0))) return rc;
+    if ((rc = radix_sort_u32(vk, it, vk1, it1, m, nullptr, bits, tmp.ptr, 0))) return rc;
   }
   Scratch dcount;
   if ((rc = ensure(dcount, (nvol + 1) * 8))) return rc;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
 #include <vector>
 
 #include "../../include/fof_gpu.h"
@@ -57,11 +58,16 @@ size_t scan_tmp_bytes(uint64_t n);
 int scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, void *tmp,
              hipStream_t st);
 
+// Passes over a list whose length an earlier kernel left in device memory
+// take the host-known capacity n and dn (the device count, n when dn is
+// null): grids are sized for the capacity and blocks past the count do no
+// work, so a call sequence needs no host round trip.
+
 // Stable LSD radix sort of (key,val) u32 pairs on bits [0,bits).  Result is
 // left in (k0,v0); (k1,v1) are ping-pong buffers of the same size.
 size_t sort_tmp_bytes(uint64_t n);
 int radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1,
-                   uint64_t n, int bits, void *tmp, hipStream_t st);
+                   uint64_t n, const uint32_t *dn, int bits, void *tmp, hipStream_t st);
 
 // out[idx[i]] = val[i] for i in [0,n), every idx[i] < nout: the pairs are
 // first partitioned (one radix pass) by the top 8 bits of idx, so that the
@@ -69,26 +75,55 @@ int radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1,
 // caches (random 4-byte writes cost one memory request each).  (idx2, val2)
 // receive the partitioned pairs; tmp: sort_tmp_bytes(n).
 int scatter_u32(uint32_t *idx, uint32_t *val, uint32_t *idx2, uint32_t *val2, uint64_t n,
-                uint64_t nout, uint32_t *out, void *tmp, hipStream_t st);
+                const uint32_t *dn, uint64_t nout, uint32_t *out, void *tmp, hipStream_t st);
 
 // Stable compaction: for every i in [0,n) with (src[i] & flag) != 0, in
 // ascending i, append key = src[i] & ~flag and value = i.  The count goes to
-// *dcount (device).  tmp: compact_tmp_bytes(n).
+// *dcount (device; must not alias dn).  tmp: compact_tmp_bytes(n).
 size_t compact_tmp_bytes(uint64_t n);
-int compact_flagged(const uint32_t *src, uint64_t n, uint32_t flag, uint32_t *out_key,
-                    uint32_t *out_val, uint32_t *dcount, void *tmp, hipStream_t st);
+int compact_flagged(const uint32_t *src, uint64_t n, const uint32_t *dn, uint32_t flag,
+                    uint32_t *out_key, uint32_t *out_val, uint32_t *dcount, void *tmp,
+                    hipStream_t st);
 
-// Device workspace owned by a context (grow-only).
-struct Work {
-  Scratch pos4, key, cnt, start, par, runend, gsize, label, sortbuf, dense, tmp, vols, vbeg,
-      small, input, out_next, out_label, out_halo, hhead, hoff, hash, fine, rtab, part;
+// p[0..n) = v (a kernel, not hipMemsetAsync: captured sequences hold kernel
+// nodes only).
+int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+
+// One pipeline: a stream, grow-only device workspace (one arena carved into
+// the call's buffers) and the host copies of what the call uploads and reads
+// back.  Kept by the context between calls.
+struct Pipe {
+  hipStream_t stream = nullptr;
+  Scratch arena;  // every buffer of a call
+  Scratch hash;   // occupied-cell hash table (sized once the cell count is known)
+  std::vector<uint8_t> vblob;  // volume table, varlink tables, volume starts
+  std::vector<uint8_t> hres;   // counts, flags and counters read back
+  hipEvent_t ev[2 * FOF_NPHASE] = {};
+  bool events = false;
+};
+
+constexpr int kMaxPipes = 8;  // concurrent pipelines of one call
+
+// Autotuning: a workload shape and the parameters chosen for it.
+struct TuneKey {
+  uint32_t lg_n, lg_nvol, sparse, varlink, ppc16, b64;
+  bool operator==(const TuneKey &o) const {
+    return lg_n == o.lg_n && lg_nvol == o.lg_nvol && sparse == o.sparse && varlink == o.varlink &&
+           ppc16 == o.ppc16 && b64 == o.b64;
+  }
+};
+struct TuneVal {
+  uint32_t T, pipes, gbig;  // dense threshold, pipelines, dense-kernel grid (0: default)
 };
 
 struct EngineState {
-  Work w;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[2 * FOF_NPHASE] = {};
-  bool events = false;
+  Pipe main;                       // pipeline 0, on the context's stream
+  std::vector<Pipe *> pipes;       // pipelines 1.. (entry 0 unused)
+  hipEvent_t fork = nullptr, join[kMaxPipes] = {}, t0 = nullptr, t1 = nullptr;
+  bool sync_events = false;
+  hipGraphExec_t graph_exec = nullptr;  // the last captured call
+  uint64_t graph_sig = 0;
+  std::vector<std::pair<TuneKey, TuneVal>> tuned;
 };
 
 }  // namespace fof
@@ -97,3 +132,11 @@ struct fof_ctx {
   int device = 0;
   fof::EngineState st;
 };
+
+// Engine entry (fof_engine.hip); arguments checked by the caller.
+extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1 *vols,
+                              uint32_t nvol, const fof_opts_v1 *opts, fof_outputs_v1 *out);
+extern "C" int fof_engine_plan(const fof_particles_v1 *P, const fof_volume_v1 *vols,
+                               uint32_t nvol, const fof_opts_v1 *opts, fof_plan_v1 *out);
+extern "C" int fof_engine_init(fof_ctx *ctx);
+extern "C" void fof_engine_free(fof_ctx *ctx);

@@ -52,11 +52,19 @@ __device__ __forceinline__ uint32_t block_reduce(uint32_t v, uint32_t *ws) {
   return t;
 }
 
+// Element count of a pass: the host-known capacity, or the smaller count
+// that an earlier kernel left in device memory (dn != nullptr), so that a
+// sequence of launches needs no host round trip between them.
+__device__ __forceinline__ uint64_t dev_count(const uint32_t *dn, uint64_t cap) {
+  return dn ? min((uint64_t)*dn, cap) : cap;
+}
+
 // ---- scan ------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restrict__ in,
-                                                        uint64_t n,
+                                                        uint64_t ncap, const uint32_t *dn,
                                                         uint32_t *__restrict__ bsum) {
   __shared__ uint32_t ws[kBlock / 64];
+  const uint64_t n = dev_count(dn, ncap);
   const uint64_t base = (uint64_t)blockIdx.x * kTile;
   uint32_t s = 0;
 #pragma unroll
@@ -71,9 +79,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restri
 // in may alias out.  boff[b] is the exclusive prefix of tile b; boff[nb] is
 // the grand total, copied to out[n] by the last block.
 __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, uint32_t *out,
-                                                      uint64_t n,
+                                                      uint64_t ncap, const uint32_t *dn,
                                                       const uint32_t *__restrict__ boff) {
   __shared__ uint32_t ws[kBlock / 64];
+  const uint64_t n = dev_count(dn, ncap);
+  if ((uint64_t)blockIdx.x * kTile > n) return;  // past the count (block sums are 0)
   const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
   uint32_t v[kItems];
   if (base + kItems <= n) {
@@ -108,7 +118,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, uint32
     for (int i = 0; i < kItems; i++)
       if (base + i < n) out[base + i] = v[i];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = boff[gridDim.x];
+  if (threadIdx.x == 0 && (uint64_t)blockIdx.x == min(n / kTile, (uint64_t)gridDim.x - 1))
+    out[n] = boff[gridDim.x];
 }
 
 __global__ void k_scan_single(uint32_t *bsum, uint32_t nb) {
@@ -133,9 +144,11 @@ __global__ void k_scan_single(uint32_t *bsum, uint32_t nb) {
 
 // ---- compaction -------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_compact_count(const uint32_t *__restrict__ src,
-                                                          uint64_t n, uint32_t flag,
+                                                          uint64_t ncap, const uint32_t *dn,
+                                                          uint32_t flag,
                                                           uint32_t *__restrict__ bsum) {
   __shared__ uint32_t ws[kBlock / 64];
+  const uint64_t n = dev_count(dn, ncap);
   const uint64_t base = (uint64_t)blockIdx.x * kTile;
   uint32_t s = 0;
 #pragma unroll
@@ -148,12 +161,14 @@ __global__ __launch_bounds__(kBlock) void k_compact_count(const uint32_t *__rest
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact_emit(const uint32_t *__restrict__ src,
-                                                         uint64_t n, uint32_t flag,
+                                                         uint64_t ncap, const uint32_t *dn,
+                                                         uint32_t flag,
                                                          const uint32_t *__restrict__ boff,
                                                          uint32_t *__restrict__ okey,
                                                          uint32_t *__restrict__ oval,
                                                          uint32_t *__restrict__ dcount) {
   __shared__ uint32_t ws[kBlock / 64];
+  const uint64_t n = dev_count(dn, ncap);
   const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
   uint32_t v[kItems], s = 0;
 #pragma unroll
@@ -175,11 +190,18 @@ __global__ __launch_bounds__(kBlock) void k_compact_emit(const uint32_t *__restr
 }
 
 // ---- radix sort -------------------------------------------------------------
+// Histograms are digit-major, hist[d * nb + b], with nb the tiles of the
+// count (device-side when dn is given); *hn receives 256 nb, the length of
+// the scan that follows.  Blocks past the count write nothing.
 __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t *__restrict__ keys,
-                                                    uint64_t n, int shift,
+                                                    uint64_t ncap, const uint32_t *dn, int shift,
                                                     uint32_t *__restrict__ hist,
-                                                    uint32_t nb) {
+                                                    uint32_t *__restrict__ hn) {
   __shared__ uint32_t h[4][256];
+  const uint64_t n = dev_count(dn, ncap);
+  const uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *hn = 256u * nb;
+  if (blockIdx.x >= nb) return;
   const int w = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < 4; q++) h[q][threadIdx.x] = 0;
@@ -204,9 +226,12 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *__restric
                                                        const uint32_t *__restrict__ vin,
                                                        uint32_t *__restrict__ kout,
                                                        uint32_t *__restrict__ vout,
-                                                       uint64_t n, int shift,
-                                                       const uint32_t *__restrict__ hoff,
-                                                       uint32_t nb) {
+                                                       uint64_t ncap, const uint32_t *dn,
+                                                       int shift,
+                                                       const uint32_t *__restrict__ hoff) {
+  const uint64_t n = dev_count(dn, ncap);
+  const uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
+  if (blockIdx.x >= nb) return;
   __shared__ uint32_t wcnt[4][256];
   __shared__ uint32_t boff[256];   // global start of each digit for this tile
   __shared__ uint32_t toff[257];   // start of each digit inside the tile
@@ -316,11 +341,26 @@ int scan_bsums(uint32_t *bs, uint64_t nb, hipStream_t st) {
   }
   uint32_t *next = bs + (nb + 1 + 63) / 64 * 64;
   uint64_t nb2 = (nb + kTile - 1) / kTile;
-  k_scan_reduce<<<(uint32_t)nb2, kBlock, 0, st>>>(bs, nb, next);
+  k_scan_reduce<<<(uint32_t)nb2, kBlock, 0, st>>>(bs, nb, nullptr, next);
   FOF_HIP(hipGetLastError());
   int rc = scan_bsums(next, nb2, st);
   if (rc) return rc;
-  k_scan_down<<<(uint32_t)nb2, kBlock, 0, st>>>(bs, bs, nb, next);
+  k_scan_down<<<(uint32_t)nb2, kBlock, 0, st>>>(bs, bs, nb, nullptr, next);
+  FOF_HIP(hipGetLastError());
+  return 0;
+}
+
+// Exclusive scan of min(*dn, n) values (dn may be null); out[count] = total.
+int scan_u32_dev(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *dn, void *tmp,
+                 hipStream_t st) {
+  if (n == 0) return fill_u32(out, 1, 0u, st);
+  uint32_t *bs = static_cast<uint32_t *>(tmp);
+  uint64_t nb = (n + kTile - 1) / kTile;
+  k_scan_reduce<<<(uint32_t)nb, kBlock, 0, st>>>(in, n, dn, bs);
+  FOF_HIP(hipGetLastError());
+  int rc = scan_bsums(bs, nb, st);
+  if (rc) return rc;
+  k_scan_down<<<(uint32_t)nb, kBlock, 0, st>>>(in, out, n, dn, bs);
   FOF_HIP(hipGetLastError());
   return 0;
 }
@@ -330,36 +370,22 @@ int scan_bsums(uint32_t *bs, uint64_t nb, hipStream_t st) {
 size_t scan_tmp_bytes(uint64_t n) { return scan_levels_bytes(n) + 256; }
 
 int scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, void *tmp, hipStream_t st) {
-  if (n == 0) {
-    FOF_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), st));
-    return 0;
-  }
-  uint32_t *bs = static_cast<uint32_t *>(tmp);
-  uint64_t nb = (n + kTile - 1) / kTile;
-  k_scan_reduce<<<(uint32_t)nb, kBlock, 0, st>>>(in, n, bs);
-  FOF_HIP(hipGetLastError());
-  int rc = scan_bsums(bs, nb, st);
-  if (rc) return rc;
-  k_scan_down<<<(uint32_t)nb, kBlock, 0, st>>>(in, out, n, bs);
-  FOF_HIP(hipGetLastError());
-  return 0;
+  return scan_u32_dev(in, out, n, nullptr, tmp, st);
 }
 
 size_t compact_tmp_bytes(uint64_t n) { return scan_levels_bytes(n) + 256; }
 
-int compact_flagged(const uint32_t *src, uint64_t n, uint32_t flag, uint32_t *okey,
-                    uint32_t *oval, uint32_t *dcount, void *tmp, hipStream_t st) {
-  if (n == 0) {
-    FOF_HIP(hipMemsetAsync(dcount, 0, sizeof(uint32_t), st));
-    return 0;
-  }
+int compact_flagged(const uint32_t *src, uint64_t n, const uint32_t *dn, uint32_t flag,
+                    uint32_t *okey, uint32_t *oval, uint32_t *dcount, void *tmp,
+                    hipStream_t st) {
+  if (n == 0) return fill_u32(dcount, 1, 0u, st);
   uint32_t *bs = static_cast<uint32_t *>(tmp);
   uint64_t nb = (n + kTile - 1) / kTile;
-  k_compact_count<<<(uint32_t)nb, kBlock, 0, st>>>(src, n, flag, bs);
+  k_compact_count<<<(uint32_t)nb, kBlock, 0, st>>>(src, n, dn, flag, bs);
   FOF_HIP(hipGetLastError());
   int rc = scan_bsums(bs, nb, st);
   if (rc) return rc;
-  k_compact_emit<<<(uint32_t)nb, kBlock, 0, st>>>(src, n, flag, bs, okey, oval, dcount);
+  k_compact_emit<<<(uint32_t)nb, kBlock, 0, st>>>(src, n, dn, flag, bs, okey, oval, dcount);
   FOF_HIP(hipGetLastError());
   return 0;
 }
@@ -367,32 +393,46 @@ int compact_flagged(const uint32_t *src, uint64_t n, uint32_t flag, uint32_t *ok
 size_t sort_tmp_bytes(uint64_t n) {
   uint64_t nb = (n + kTile - 1) / kTile;
   uint64_t h = 256 * nb;
-  return (h + 64) * sizeof(uint32_t) + scan_tmp_bytes(h) + 256;
+  return (h + 64 + 64) * sizeof(uint32_t) + scan_tmp_bytes(h) + 256;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_pairs(uint32_t *__restrict__ k0,
+                                                       uint32_t *__restrict__ v0,
+                                                       const uint32_t *__restrict__ k1,
+                                                       const uint32_t *__restrict__ v1,
+                                                       uint64_t ncap, const uint32_t *dn) {
+  const uint64_t n = dev_count(dn, ncap);
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kBlock) {
+    k0[i] = k1[i];
+    v0[i] = v1[i];
+  }
 }
 
 int radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint64_t n,
-                   int bits, void *tmp, hipStream_t st) {
+                   const uint32_t *dn, int bits, void *tmp, hipStream_t st) {
   if (n <= 1 || bits <= 0) return 0;
   const uint64_t nb = (n + kTile - 1) / kTile;
   uint32_t *hist = static_cast<uint32_t *>(tmp);
-  void *stmp = hist + 256 * nb + 64;
+  uint32_t *hn = hist + 256 * nb + 64;  // histogram length, set on the device
+  void *stmp = hn + 64;
   uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
   int passes = 0;
   for (int shift = 0; shift < bits; shift += 8, passes++) {
-    k_rs_hist<<<(uint32_t)nb, kBlock, 0, st>>>(ka, n, shift, hist, (uint32_t)nb);
+    k_rs_hist<<<(uint32_t)nb, kBlock, 0, st>>>(ka, n, dn, shift, hist, hn);
     FOF_HIP(hipGetLastError());
-    int rc = scan_u32(hist, hist, 256 * nb, stmp, st);
+    int rc = scan_u32_dev(hist, hist, 256 * nb, hn, stmp, st);
     if (rc) return rc;
-    k_rs_scatter<<<(uint32_t)nb, kBlock, 0, st>>>(ka, va, kb, vb, n, shift, hist,
-                                                  (uint32_t)nb);
+    k_rs_scatter<<<(uint32_t)nb, kBlock, 0, st>>>(ka, va, kb, vb, n, dn, shift, hist);
     FOF_HIP(hipGetLastError());
     uint32_t *t = ka;
     ka = kb, kb = t;
     t = va, va = vb, vb = t;
   }
   if (passes & 1) {
-    FOF_HIP(hipMemcpyAsync(k0, ka, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-    FOF_HIP(hipMemcpyAsync(v0, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    k_copy_pairs<<<(uint32_t)std::min<uint64_t>(nb * 4, 65536), kBlock, 0, st>>>(k0, v0, ka, va, n,
+                                                                                 dn);
+    FOF_HIP(hipGetLastError());
   }
   return 0;
 }
@@ -402,8 +442,9 @@ int radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint6
 // cover one stretch of the partitioned list (a grid-stride loop would spread
 // them over many partitions).
 __global__ __launch_bounds__(kBlock) void k_put(const uint32_t *__restrict__ idx,
-                                                const uint32_t *__restrict__ val, uint64_t n,
-                                                uint32_t *__restrict__ out) {
+                                                const uint32_t *__restrict__ val, uint64_t ncap,
+                                                const uint32_t *dn, uint32_t *__restrict__ out) {
+  const uint64_t n = dev_count(dn, ncap);
   const uint64_t base = (uint64_t)blockIdx.x * kTile;
 #pragma unroll
   for (int r = 0; r < kItems; r++) {
@@ -413,22 +454,50 @@ __global__ __launch_bounds__(kBlock) void k_put(const uint32_t *__restrict__ idx
 }
 
 int scatter_u32(uint32_t *idx, uint32_t *val, uint32_t *idx2, uint32_t *val2, uint64_t n,
-                uint64_t nout, uint32_t *out, void *tmp, hipStream_t st) {
+                const uint32_t *dn, uint64_t nout, uint32_t *out, void *tmp, hipStream_t st) {
   if (n == 0) return 0;
   int bits = 0;
   while (bits < 32 && (nout - 1) >> bits) bits++;
   const int shift = bits > 8 ? bits - 8 : 0;
   const uint64_t nb = (n + kTile - 1) / kTile;
   uint32_t *hist = static_cast<uint32_t *>(tmp);
-  void *stmp = hist + 256 * nb + 64;
-  k_rs_hist<<<(uint32_t)nb, kBlock, 0, st>>>(idx, n, shift, hist, (uint32_t)nb);
+  uint32_t *hn = hist + 256 * nb + 64;
+  void *stmp = hn + 64;
+  k_rs_hist<<<(uint32_t)nb, kBlock, 0, st>>>(idx, n, dn, shift, hist, hn);
   FOF_HIP(hipGetLastError());
-  int rc = scan_u32(hist, hist, 256 * nb, stmp, st);
+  int rc = scan_u32_dev(hist, hist, 256 * nb, hn, stmp, st);
   if (rc) return rc;
-  k_rs_scatter<<<(uint32_t)nb, kBlock, 0, st>>>(idx, val, idx2, val2, n, shift, hist,
-                                                (uint32_t)nb);
+  k_rs_scatter<<<(uint32_t)nb, kBlock, 0, st>>>(idx, val, idx2, val2, n, dn, shift, hist);
   FOF_HIP(hipGetLastError());
-  k_put<<<(uint32_t)nb, kBlock, 0, st>>>(idx2, val2, n, out);
+  k_put<<<(uint32_t)nb, kBlock, 0, st>>>(idx2, val2, n, dn, out);
+  FOF_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---- fills ------------------------------------------------------------------
+// Kernels rather than hipMemsetAsync, so that a captured call sequence holds
+// kernel nodes only.
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *__restrict__ p, uint64_t n,
+                                                     uint32_t v, int vec) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  uint64_t done = 0;
+  if (vec) {  // p is 16-byte aligned
+    const uint64_t n4 = n / 4;
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+    const uint4 w = make_uint4(v, v, v, v);
+    for (uint64_t i = t; i < n4; i += stride) q[i] = w;
+    done = n4 * 4;
+  }
+  for (uint64_t i = done + t; i < n; i += stride) p[i] = v;
+}
+
+int fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) {
+  if (n == 0) return 0;
+  const int vec = (reinterpret_cast<uintptr_t>(p) & 15u) == 0 ? 1 : 0;
+  const uint64_t per = vec ? 4 : 1;
+  const uint64_t blocks = std::min<uint64_t>((n / per + kBlock - 1) / kBlock + 1, 16384);
+  k_fill_u32<<<(uint32_t)blocks, kBlock, 0, st>>>(p, n, v, vec);
   FOF_HIP(hipGetLastError());
   return 0;
 }

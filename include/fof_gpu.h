@@ -35,8 +35,11 @@ extern "C" {
 #define FOF_E_DOMAIN (-2) /* particle outside [rmin_buff,rmax_buff] or non-finite:
                              the reference would index first[] out of bounds
                              (src_v2/FOF.c:79-85)                              */
-#define FOF_E_RANGE (-3)  /* n >= 2^32-1, cell count >= 2^32-1, or an output
-                             capacity too small                                */
+#define FOF_E_RANGE (-3)  /* n >= 2^31-1 particles in one call (labels carry a
+                             flag in bit 31; the reference's u32 indices,
+                             src_v2/FOF.c:29, allow 2^32-2), a grid too large
+                             for 64-bit cell keys, or an output capacity too
+                             small                                             */
 #define FOF_E_NOMEM (-4)  /* device allocation failed                          */
 #define FOF_E_HIP (-5)    /* HIP runtime error                                 */
 #define FOF_E_NODEV (-6)  /* no usable gfx950 device                           */
@@ -95,11 +98,21 @@ typedef struct fof_domain_v1 {
                                    pointers                                   */
 #define FOF_OPT_NO_RING 0x4u    /* do not write next_out                      */
 #define FOF_OPT_NO_HALOS 0x8u   /* skip the halo table                        */
-#define FOF_OPT_SYNC_ONLY 0x10u /* reserved                                   */
+#define FOF_OPT_GRAPH 0x10u    /* capture the call's launch sequence in a HIP
+                                   graph and replay it on later calls with
+                                   identical arguments (device-resident input
+                                   and outputs, constant linking length, dense
+                                   cell addressing; other calls run eagerly)  */
 #define FOF_OPT_SPARSE 0x20u    /* address only occupied cells (sorted keys +
                                    hash table); chosen automatically when a
                                    dense cell table would be much larger than
-                                   the particle count                         */
+                                   the particle count or reach 2^31 cells     */
+#define FOF_OPT_AUTOTUNE 0x40u  /* pick the dense-cell threshold and the number
+                                   of concurrent pipelines for this workload
+                                   shape by timing candidates on the call's own
+                                   input (once per shape; results are cached
+                                   per context and in the file named by the
+                                   environment variable FOF_TUNE_CACHE)       */
 
 typedef struct fof_opts_v1 {
   uint32_t version;
@@ -109,6 +122,12 @@ typedef struct fof_opts_v1 {
   uint32_t dense_threshold; /* cells with at least this many particles are
                                linked at sub-cell level; 0 = default (24),
                                values above 512 act as 512                   */
+  uint32_t pipelines;       /* a batch is split over this many concurrent
+                               pipelines (volumes in contiguous groups of
+                               about equal particle counts, each on its own
+                               stream and workspace); 0 or 1: one, at most 8 */
+  uint32_t big_grid;        /* workgroups of the 256-thread dense-cell kernels;
+                               0 = default (8192)                           */
   uint32_t reserved;
 } fof_opts_v1;
 
@@ -161,6 +180,15 @@ typedef struct fof_stats_v1 {
   uint32_t dense_threshold; /* the dense-cell threshold in effect            */
   double phase_ms[FOF_NPHASE];
   double link_kernel_ms;  /* spans 5 + 6 + 15: the pair-test kernels         */
+  uint64_t pair_tests_kernel[3]; /* pair tests by kernel: sparse cells, dense
+                                    cells of 256 particles or more, smaller
+                                    dense cells (sum = pair_tests)            */
+  uint64_t pairs_joined;  /* candidate run pairs not tested because both ends
+                             were already in one set                         */
+  uint32_t pipelines;     /* concurrent pipelines the volumes were split over */
+  uint32_t graph;         /* 1: replayed a captured graph, 2: captured one    */
+  uint32_t big_grid;      /* workgroups of the 256-thread dense kernels       */
+  uint32_t reserved;
 } fof_stats_v1;
 
 typedef struct fof_ctx fof_ctx;
@@ -202,6 +230,63 @@ int fof_link_gpu_batch(fof_ctx *ctx, const fof_particles_v1 *particles,
                        uint32_t *label_out, fof_halo_v1 *halos,
                        uint64_t halo_cap, uint64_t *halo_offsets,
                        fof_stats_v1 *stats);
+
+/* Second moments of a halo (INERTIA / DISPERSION, src_v2/FOF.c:848-872).
+ * pos2 / vel2: FP64 sums over the members, in the ring order of fof_halo_v1's
+ * sums, of the products {xx, yy, zz, xy, xz, yz} of the float Pos / Vel
+ * widened to double: the reference's accumulators xh2, yh2, zh2, xhyh, xhzh,
+ * yhzh (and vxh2 ... vyhvzh) bit for bit, from which its single-pass
+ * formulas follow.  cpos2 / cvel2: the same sums taken about the mean,
+ * sum (a - <a>)(b - <b>), accumulated about the head particle and recentred,
+ * so without the cancellation of the single-pass form.  The reference's
+ * inertia tensor is {cyy + czz, cxx + czz, cxx + cyy, -cxy, -cxz, -cyz} and
+ * its dispersion tensor cvel2 / n. */
+typedef struct fof_moments_v1 {
+  double pos2[6], vel2[6];
+  double cpos2[6], cvel2[6];
+} fof_moments_v1;
+
+/* Outputs of fof_link_gpu_ex.  Every array lives where `location` says;
+ * NULL skips an output. */
+typedef struct fof_outputs_v1 {
+  uint32_t version;        /* FOF_ABI_VERSION                                 */
+  uint32_t location;       /* FOF_MEM_HOST or FOF_MEM_DEVICE                  */
+  uint32_t *next;          /* [n] successor ring, as next_out above          */
+  uint32_t *label;         /* [n] group minimum index (volume-local)         */
+  uint64_t *minid;         /* [n] minimum particle ID of the group: the
+                              canonical label, independent of input order
+                              (needs off_id >= 0)                            */
+  fof_halo_v1 *halos;      /* [halo_cap] halos, volume after volume          */
+  fof_moments_v1 *moments; /* [halo_cap] second moments of the same halos    */
+  uint64_t halo_cap;
+  uint64_t *halo_offsets;  /* host [nvol + 1]: first halo of each volume     */
+  uint64_t nhalos;         /* out: halos found (also with FOF_E_RANGE, when
+                              halo_cap was too small)                        */
+  fof_stats_v1 *stats;     /* host, may be NULL                              */
+} fof_outputs_v1;
+
+/* Link a batch of volumes (one volume: nvol = 1) with every output the
+ * engine computes.  fof_link_gpu and fof_link_gpu_batch are this call with
+ * FOF_OPT_OUT_DEVICE selecting the outputs' location. */
+int fof_link_gpu_ex(fof_ctx *ctx, const fof_particles_v1 *particles,
+                    const fof_volume_v1 *vols, uint32_t nvol,
+                    const fof_opts_v1 *opts, fof_outputs_v1 *out);
+
+/* What a call would do, computed on the host (no device needed): the cell
+ * addressing chosen, the grid sizes and the device workspace it needs. */
+typedef struct fof_plan_v1 {
+  uint32_t version;
+  uint32_t sparse;          /* 1: occupied-cell addressing                   */
+  uint64_t n;               /* particles                                     */
+  uint64_t cells;           /* dense cell table entries (0 when sparse)      */
+  double grid_cells;        /* cells of the reference grids, all volumes     */
+  uint64_t workspace_bytes; /* device workspace of the call                  */
+  uint32_t dense_threshold; /* in effect                                     */
+  uint32_t reserved;
+} fof_plan_v1;
+
+int fof_plan(const fof_particles_v1 *particles, const fof_volume_v1 *vols,
+             uint32_t nvol, const fof_opts_v1 *opts, fof_plan_v1 *out);
 
 /* Text for an error code. */
 const char *fof_strerror(int code);

@@ -102,6 +102,10 @@ def lib():
         L.or_retrieve.argtypes = [C.c_uint32, u32p, f32p, f32p, dp, vp, C.c_uint64]
         L.or_retrieve.restype = C.c_int64
         L.or_props_comp.argtypes = [C.c_uint32, u32p, f32p, f32p, C.c_uint64, u32p, f64p, f64p]
+        L.or_moments.argtypes = [C.c_uint32, u32p, f32p, f32p, C.c_uint64, u32p, f64p, f64p]
+        L.or_moments_reference_out.argtypes = [C.c_uint32, f64p, f64p, f64p,
+                                               np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS"),
+                                               np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
         _lib = L
     return _lib
 
@@ -236,6 +240,39 @@ def props_compensated(label0, pos, vel, heads):
     sv = np.zeros((len(h), 3), np.float64)
     lib().or_props_comp(len(lab), lab, _pos(pos), _pos(vel), len(h), h, sp.reshape(-1), sv.reshape(-1))
     return sp, sv
+
+
+def moments(next0, pos, vel, heads):
+    """Second moments of the halos headed by `heads` over the ring next0
+    (src_v2/FOF.c:811-824): (raw, centred), each (nh, 12) -- Pos
+    {xx,yy,zz,xy,xz,yz} then Vel; raw in ring order, centred compensated."""
+    nxt = np.ascontiguousarray(next0, dtype=np.uint32)
+    h = np.ascontiguousarray(heads, dtype=np.uint32)
+    raw = np.zeros((len(h), 12), np.float64)
+    cen = np.zeros((len(h), 12), np.float64)
+    lib().or_moments(len(nxt), nxt, _pos(pos), _pos(vel), len(h), h, raw.reshape(-1), cen.reshape(-1))
+    return raw, cen
+
+
+def moments_reference_out(n, sum_pos, sum_vel, raw):
+    """The reference's written INERTIA and DISPERSION floats for one halo
+    (src_v2/FOF.c:848-860)."""
+    ine = np.zeros(6, np.float32)
+    dis = np.zeros(6, np.float32)
+    lib().or_moments_reference_out(int(n), np.ascontiguousarray(sum_pos, np.float64),
+                                   np.ascontiguousarray(sum_vel, np.float64),
+                                   np.ascontiguousarray(raw, np.float64), ine, dis)
+    return ine, dis
+
+
+def minid_labels(label0, ids):
+    """Canonical labels: every particle's group minimum particle ID (the
+    relabelling under which membership is compared, independent of order)."""
+    lab = np.asarray(label0, dtype=np.int64)
+    ids = np.asarray(ids, dtype=np.uint64)
+    m = np.full(len(lab), np.iinfo(np.uint64).max, np.uint64)
+    np.minimum.at(m, lab, ids)
+    return m[lab]
 
 
 def canonical(label0):

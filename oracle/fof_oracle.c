@@ -421,3 +421,81 @@ void or_props_comp(uint32_t n, const uint32_t *label0, const float *pos,
     }
   free(slot), free(acc);
 }
+
+/* Second moments of halos (src_v2/FOF.c:811-824: INERTIA / DISPERSION
+ * accumulators xh2, yh2, zh2, xhyh, xhzh, yhzh and vxh2 ... vyhvzh).
+ * raw[12k + q]: FP64 sums in ring order from heads[k] of the products
+ * (double)a * (double)b of the float members, q = 0-5 Pos {xx,yy,zz,xy,xz,yz},
+ * 6-11 Vel.  cen[12k + q]: the centred sums sum (a - <a>)(b - <b>), from a
+ * compensated mean and a compensated second pass. */
+static const int mom_a[6] = {0, 1, 2, 0, 0, 1}, mom_b[6] = {0, 1, 2, 1, 2, 2};
+
+void or_moments(uint32_t n, const uint32_t *next0, const float *pos,
+                const float *vel, uint64_t nh, const uint32_t *heads,
+                double *raw, double *cen) {
+  for (uint64_t k = 0; k < nh; k++) {
+    const uint32_t i = heads[k];
+    double r[12] = {0};
+    double s[6] = {0}, c[6] = {0};
+    uint32_t j = i;
+    uint32_t np = 0;
+    do {
+      const float *p = &pos[3 * j], *v = &vel[3 * j];
+      for (int q = 0; q < 6; q++) {
+        r[q] += (double)p[mom_a[q]] * (double)p[mom_b[q]];
+        r[6 + q] += (double)v[mom_a[q]] * (double)v[mom_b[q]];
+      }
+      for (int q = 0; q < 3; q++) {
+        neu_add(&s[q], &c[q], p[q]);
+        neu_add(&s[3 + q], &c[3 + q], v[q]);
+      }
+      np++;
+      j = next0[j];
+    } while (j != i && j < n);
+    double m[6];
+    for (int q = 0; q < 6; q++) m[q] = (s[q] + c[q]) / (double)np;
+    double t[12] = {0}, tc[12] = {0};
+    j = i;
+    do {
+      const float *p = &pos[3 * j], *v = &vel[3 * j];
+      for (int q = 0; q < 6; q++) {
+        neu_add(&t[q], &tc[q], ((double)p[mom_a[q]] - m[mom_a[q]]) * ((double)p[mom_b[q]] - m[mom_b[q]]));
+        neu_add(&t[6 + q], &tc[6 + q],
+                ((double)v[mom_a[q]] - m[3 + mom_a[q]]) * ((double)v[mom_b[q]] - m[3 + mom_b[q]]));
+      }
+      j = next0[j];
+    } while (j != i && j < n);
+    for (int q = 0; q < 12; q++) {
+      raw[12 * k + q] = r[q];
+      cen[12 * k + q] = t[q] + tc[q];
+    }
+  }
+}
+
+/* The values the reference writes for a halo with INERTIA and DISPERSION
+ * (src_v2/FOF.c:848-860, stored as float): inertia {Ixx, Iyy, Izz, Ixy, Ixz,
+ * Iyz} from the raw sums with its single-pass formulas, and the velocity
+ * dispersion tensor {xx, yy, zz, xy, xz, yz}. */
+void or_moments_reference_out(uint32_t np, const double *sum_pos,
+                              const double *sum_vel, const double *raw,
+                              float *inertia, float *disp) {
+  const double inv = 1.0 / (double)np;
+  const double xh = sum_pos[0], yh = sum_pos[1], zh = sum_pos[2];
+  const double vxh = sum_vel[0], vyh = sum_vel[1], vzh = sum_vel[2];
+  const double xh2 = raw[0], yh2 = raw[1], zh2 = raw[2], xhyh = raw[3], xhzh = raw[4],
+               yhzh = raw[5];
+  const double vxh2 = raw[6], vyh2 = raw[7], vzh2 = raw[8], vxhvyh = raw[9],
+               vxhvzh = raw[10], vyhvzh = raw[11];
+  inertia[0] = (float)(yh2 + zh2 - inv * (yh * yh + zh * zh));
+  inertia[1] = (float)(xh2 + zh2 - inv * (xh * xh + zh * zh));
+  inertia[2] = (float)(xh2 + yh2 - inv * (xh * xh + yh * yh));
+  inertia[3] = (float)(inv * xh * yh - xhyh);
+  inertia[4] = (float)(inv * xh * zh - xhzh);
+  inertia[5] = (float)(inv * yh * zh - yhzh);
+  disp[0] = (float)((vxh2 - vxh * vxh * inv) * inv);
+  disp[1] = (float)((vyh2 - vyh * vyh * inv) * inv);
+  disp[2] = (float)((vzh2 - vzh * vzh * inv) * inv);
+  disp[3] = (float)((vxhvyh - vxh * vyh * inv) * inv);
+  disp[4] = (float)((vxhvzh - vxh * vzh * inv) * inv);
+  disp[5] = (float)((vyhvzh - vyh * vzh * inv) * inv);
+}

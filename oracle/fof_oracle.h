@@ -21,6 +21,11 @@
  *   or_retrieve        closed forms (lattice clumps, symmetric groups),
  *                      heads = minimum index, size cut, ownership boundaries
  *   or_props_comp      compensated sums vs exact rational sums in tests
+ *   or_moments         closed forms (axis-symmetric clusters), exact
+ *                      rational sums (raw in ring order to FP64 rounding,
+ *                      centred to a few ulps)
+ *   or_moments_reference_out  the reference's formulas on hand-checked
+ *                      small cases (exact in binary fractions)
  * The reference itself cannot be compiled here (vars.h includes GSL headers
  * the image lacks), so no golden output of the reference exists.
  */
@@ -91,6 +96,19 @@ int64_t or_retrieve(uint32_t n, const uint32_t *next0, const float *pos,
 void or_props_comp(uint32_t n, const uint32_t *label0, const float *pos,
                    const float *vel, uint64_t nh, const uint32_t *heads,
                    double *sum_pos, double *sum_vel);
+
+/* Second moments over a ring (src_v2/FOF.c:811-824), raw[12*nh] in ring
+ * order and centred cen[12*nh] (compensated two-pass); q = 0-5 Pos
+ * {xx,yy,zz,xy,xz,yz}, 6-11 Vel. */
+void or_moments(uint32_t n, const uint32_t *next0, const float *pos,
+                const float *vel, uint64_t nh, const uint32_t *heads,
+                double *raw, double *cen);
+
+/* The reference's written INERTIA / DISPERSION values (src_v2/FOF.c:848-860)
+ * from the FP64 sums and raw second moments of one halo. */
+void or_moments_reference_out(uint32_t np, const double *sum_pos,
+                              const double *sum_vel, const double *raw,
+                              float *inertia, float *disp);
 
 #ifdef __cplusplus
 }

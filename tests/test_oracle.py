@@ -233,3 +233,89 @@ def test_varlink_cell_rule():
     nx, _ = O.link_ring(pos, d2, vl2)
     assert np.array_equal(lb, lu) and np.array_equal(O.labels_from_ring(nx), lu)
     assert len(np.unique(lu)) < len(lu)
+
+
+# ---- second moments (src_v2/FOF.c:811-824, :848-860) -----------------------
+def test_moments_closed_form():
+    # +-a along each axis about c: centred xx,yy,zz = 2a_k^2, cross terms 0;
+    # raw sums = n c_a c_b + centred (c and a exact binary fractions)
+    c = np.array([250.5, -17.25, 1000.125])
+    a = [1.5, 0.25, 3.0]
+    pos = []
+    for k in range(3):
+        for s in (1, -1):
+            p = c.copy()
+            p[k] += s * a[k]
+            pos.append(p)
+    pos = np.array(pos, np.float32)
+    vel = (pos - c.astype(np.float32)) * np.float32(2.0)
+    lab = np.zeros(6, np.uint32)
+    raw, cen = O.moments(O.ring_from_labels(lab), pos, vel, [0])
+    exp_c = [2 * a[0] ** 2, 2 * a[1] ** 2, 2 * a[2] ** 2, 0.0, 0.0, 0.0]
+    assert np.array_equal(cen[0, :6], exp_c)
+    assert np.array_equal(cen[0, 6:], 4 * np.array(exp_c))
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    for q, (i, j) in enumerate(pairs):
+        assert raw[0, q] == 6 * c[i] * c[j] + exp_c[q]
+
+
+def test_moments_vs_exact_rationals():
+    # raw: FP64 sums in ring order agree with the exact sums to FP64 rounding;
+    # centred: the compensated two-pass value is the exact one to a few ulps
+    from fractions import Fraction as Fr
+
+    rng = np.random.default_rng(17)
+    n = 600
+    grp = rng.integers(0, 12, n)
+    first = np.full(12, n)
+    np.minimum.at(first, grp, np.arange(n))
+    lab = first[grp].astype(np.uint32)  # group minimum index, as the linkers label
+    pos = (500.0 + rng.normal(0, 0.3, (n, 3))).astype(np.float32)
+    vel = rng.normal(0, 300, (n, 3)).astype(np.float32)
+    heads = np.unique(lab)
+    raw, cen = O.moments(O.ring_from_labels(lab), pos, vel, heads)
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    for k, h in enumerate(heads):
+        m = np.flatnonzero(lab == h)
+        for blk, arr in ((0, pos), (1, vel)):
+            X = [[Fr(float(arr[i, d])) for d in range(3)] for i in m]
+            mean = [sum(x[d] for x in X) / len(X) for d in range(3)]
+            for q, (i, j) in enumerate(pairs):
+                exact_raw = sum(x[i] * x[j] for x in X)
+                exact_cen = sum((x[i] - mean[i]) * (x[j] - mean[j]) for x in X)
+                scale = float(abs(exact_raw)) or 1.0
+                assert abs(raw[k, 6 * blk + q] - float(exact_raw)) <= 4 * len(m) * 1.2e-16 * scale
+                cs = float(sum(abs(x[i] - mean[i]) * abs(x[j] - mean[j]) for x in X)) or 1.0
+                assert abs(cen[k, 6 * blk + q] - float(exact_cen)) <= 1e-14 * cs
+
+
+def test_moments_reference_formulas():
+    # two particles at x = 1 and 3 (y = z = 0), velocities 2 and 6 on x:
+    # sums xh = 4, xh2 = 10 -> Iyy = Izz = xh2 - xh^2/2 = 2, Ixx = 0; dispersion
+    # vx: (40 - 64/2)/2 = 4
+    sp = np.array([4.0, 0, 0])
+    sv = np.array([8.0, 0, 0])
+    raw = np.zeros(12)
+    raw[0] = 10.0
+    raw[6] = 40.0
+    ine, dis = O.moments_reference_out(2, sp, sv, raw)
+    assert list(ine) == [0.0, 2.0, 2.0, 0.0, 0.0, 0.0]
+    assert list(dis) == [4.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+
+
+def test_minid_labels_order_free():
+    # canonical labels (group minimum particle ID) do not depend on the input
+    # order, while minimum-index labels do
+    rng = np.random.default_rng(23)
+    pos = random_set(23, n=3000, L=12.0)
+    ids = rng.permutation(10 ** 6)[:3000].astype(np.uint64)
+    d = dom(0.0, 12.0, 12 / np.cbrt(3000), 0.3)
+    lab, _ = O.link_uf(pos, d)
+    can = O.minid_labels(lab, ids)
+    perm = rng.permutation(3000)
+    lab2, _ = O.link_uf(pos[perm], d)
+    can2 = O.minid_labels(lab2, ids[perm])
+    assert np.array_equal(can[perm], can2)
+    for g in np.unique(lab)[:50]:
+        m = lab == g
+        assert np.all(can[m] == ids[m].min())

@@ -323,7 +323,7 @@ def cpu_baseline(recs_u8, bx, pool, ncores, ctx, L, budget_s=12.0):
     pending = []
     nsub = 0
     while time.perf_counter() - t_wall < budget_s or pending:
-        while len(pending) < 2 * ncores and time.perf_counter() - t_wall < budget_s and nsub < 512:
+        while len(pending) < ncores and time.perf_counter() - t_wall < budget_s and nsub < 512:
             c = rng.uniform(S / 2 + shell + 1, bx.box - S / 2 - shell - 1, 3)
             lo, hi = c - S / 2 - shell, c + S / 2 + shell
             sub, _ = subbox(recs_u8, xyz, lo, hi)

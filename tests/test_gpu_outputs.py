@@ -218,3 +218,33 @@ def test_lattice_at_linking_length(ctx, below):
     assert len(np.unique(olab)) == (len(P) if below else 1)
     if not below:
         assert st.rechecks > 0
+
+
+@pytest.mark.parametrize("fmt", ["ascii", "gadget1", "picola"])
+def test_read_format_ghosts_link(ctx, fmt):
+    # a fixture in each of the reference's input formats: read on the host,
+    # ghost shell built on the device, linked, checked against the oracle
+    import os
+
+    import torch
+
+    from cm_halofinder_amd import formats as F
+
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    path = os.path.join(gold, f"particles.{fmt}")
+    recs = {"ascii": lambda: F.read_ascii(path), "gadget1": lambda: F.read_gadget1(path)[0],
+            "picola": lambda: F.read_picola(path)}[fmt]()
+    assert recs.tobytes() == np.load(os.path.join(gold, "expected.npy"), allow_pickle=False).tobytes()
+    bx = W.plain(10, seed=71)
+    gh = L.ghost_params(bx.box, 2.0, 1, 1, 1, bx.pside, bx.linklength, 5)
+    drecs = torch.from_numpy(recs.view(np.uint8).copy()).cuda()
+    counts, vols = L.ghosts_device(gh, drecs, len(recs))
+    tot = int(counts.sum())
+    dout = torch.empty(tot * 32, dtype=torch.uint8, device="cuda")
+    counts, vols = L.ghosts_device(gh, drecs, len(recs), dout, tot)
+    P = dout.cpu().numpy().view(L.PART_DTYPE)
+    hP, _, _ = L.ghosts_host(gh, recs)
+    assert np.array_equal(np.sort(P, order=["ID", "Pos"]), np.sort(hP, order=["ID", "Pos"]))
+    rc, nh, nxt, lab, mid, halos, mom, _, _ = link_ex(ctx, P, vols)
+    L.check(rc)
+    check_volume(P, vols[0], lab, nxt, mid, halos[:nh], mom[:nh])

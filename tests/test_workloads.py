@@ -128,3 +128,28 @@ def test_lightcone_device_generator_and_shells():
         a = np.sort(Dq[int(v.offset):int(v.offset + v.n)]["ID"])
         b = np.sort(Q[int(w.offset):int(w.offset + w.n)]["ID"])
         assert np.array_equal(a, b)
+
+
+def test_ghost_shell_complete():
+    # every periodic image (x + s box, s in {-1,0,1} per axis, rounded to
+    # float as src_v2/read_data.c:792-797 does) that lies in a volume's
+    # buffered box is present in that volume, and nothing else is: the
+    # builder's output equals a brute-force enumeration, as multisets
+    bx = W.plain(12, seed=3)
+    recs = L.gen_host(bx.gen())
+    pos = recs["Pos"].astype(np.float64)
+    shifts = np.array([(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)], np.float64)
+    imgs = (pos[:, None, :] + shifts[None, :, :] * bx.box).astype(np.float32)  # (n, 27, 3)
+    ids = np.repeat(recs["ID"], 27)
+    flat = imgs.reshape(-1, 3).astype(np.float64)
+    for g in (1, 2, 3):
+        out, counts, vols = L.ghosts_host(bx.ghost(g), recs)
+        for v in vols:
+            lo, hi = np.array(v.domain.rmin_buff[:]), np.array(v.domain.rmax_buff[:])
+            m = np.all((flat >= lo) & (flat <= hi), 1)
+            want = np.rec.fromarrays([ids[m], flat[m, 0].astype(np.float32), flat[m, 1].astype(np.float32),
+                                      flat[m, 2].astype(np.float32)])
+            P = out[int(v.offset):int(v.offset + v.n)]
+            got = np.rec.fromarrays([P["ID"], P["Pos"][:, 0], P["Pos"][:, 1], P["Pos"][:, 2]])
+            assert len(got) == len(want), (g, len(got), len(want))
+            assert np.array_equal(np.sort(got), np.sort(want))

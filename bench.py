@@ -463,6 +463,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-volumes", action="store_true", help="skip the volumes block of the box line")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="time only the batched call (for profiling runs): no plain path, parity "
+                         "sample, CPU baseline, PCIe or volumes")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default configuration, eager launches (no autotuning, no graph)")
     ap.add_argument("--workload", default="box", choices=["box", "precision", "volumes", "lightcone"],
                     help="box: the headline (1024^3, plain vs batched, plus the 4096-volume batch); "
                          "precision: the same box with 1%% of particles in borderline pairs; "
@@ -479,6 +484,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.only_batched:
+        args.no_cpu = args.no_parity = args.no_volumes = True
     pool, ncores = (None, 0) if (args.no_cpu or args.workload != "box") else cpu_pool()
 
     import torch
@@ -538,7 +545,7 @@ def main():
     hoff = np.zeros(len(VB) + 1, np.uint64)
     # timed calls: autotuned configuration, replayed as a captured graph;
     # per-phase times come from one eager call with the same configuration
-    flags = L.OPT_OUT_DEVICE | L.OPT_AUTOTUNE | L.OPT_GRAPH
+    flags = L.OPT_OUT_DEVICE if args.no_tune else L.OPT_OUT_DEVICE | L.OPT_AUTOTUNE | L.OPT_GRAPH
     st1, stB = L.Stats(), L.Stats()
     P1, PB = L.particles(D1, n=N1), L.particles(DB, n=NB)
     dom1 = V1[0].domain
@@ -553,15 +560,18 @@ def main():
         rc = ctx.link_batch(PB, VB, L.opts(fl), nxt, lab, hal, cap, hoff, st)
         L.check(rc, "batched")
 
-    t0 = time.perf_counter()
-    step_plain()
-    torch.cuda.synchronize()
-    log(f"plain: autotuned in {time.perf_counter() - t0:.1f}s")
-    t_plain = run_steps(step_plain, args.warmup, args.steps)
-    step_plain(L.OPT_OUT_DEVICE | L.OPT_AUTOTUNE | L.OPT_TIMING, st1)
-    log(f"plain: {t_plain * 1e3:.1f} ms/step (T={st1.dense_threshold})  phases "
-        f"{dict(zip(L.PHASES, [round(x, 2) for x in st1.phase_ms]))}")
-    fp1 = group_fingerprints(D1, N1, lab, hal, nh1[0])
+    t_plain = float("nan")
+    fp1 = None
+    if not args.only_batched:
+        t0 = time.perf_counter()
+        step_plain()
+        torch.cuda.synchronize()
+        log(f"plain: first call (autotuning) {time.perf_counter() - t0:.1f}s")
+        t_plain = run_steps(step_plain, args.warmup, args.steps)
+        step_plain((flags & ~L.OPT_GRAPH) | L.OPT_TIMING, st1)
+        log(f"plain: {t_plain * 1e3:.1f} ms/step (T={st1.dense_threshold})  phases "
+            f"{dict(zip(L.PHASES, [round(x, 2) for x in st1.phase_ms]))}")
+        fp1 = group_fingerprints(D1, N1, lab, hal, nh1[0])
     parity = None
     if not args.no_parity:
         t0 = time.perf_counter()
@@ -577,7 +587,7 @@ def main():
     t0 = time.perf_counter()
     step_batch()
     torch.cuda.synchronize()
-    log(f"batched: autotuned in {time.perf_counter() - t0:.1f}s")
+    log(f"batched: first call (autotuning) {time.perf_counter() - t0:.1f}s")
     t_batch = run_steps(step_batch, args.warmup, args.steps)
     stG = L.Stats()
     step_batch(flags, stG)  # the timed configuration (graph replay)
@@ -591,20 +601,23 @@ def main():
     nhB = int(hoff[-1])
     fpB = group_fingerprints(DB, NB, lab, hal, nhB, [int(v.offset) for v in VB], [int(v.n) for v in VB],
                              hoff)
-    same_groups = bool(fp1.shape == fpB.shape and torch.equal(fp1, fpB))
-    log(f"owned halos plain {fp1.shape[0]} batched {fpB.shape[0]} identical membership: {same_groups}")
+    same_groups = fp1 is None or bool(fp1.shape == fpB.shape and torch.equal(fp1, fpB))
+    if fp1 is not None:
+        log(f"owned halos plain {fp1.shape[0]} batched {fpB.shape[0]} identical membership: {same_groups}")
 
     cellsB = L.plan(PB, VB).cells
     wl = "box_1024" if args.workload == "box" and args.pside == 1024 and args.g == 4 else None
     roof = roofline(stB, L, DB, VB, NB, cellsB, wl)
     log("roofline:", roof)
-    h2d, d2h = pcie_rates(dev)
-    pcie = {"h2d_GBs": h2d / 1e9, "d2h_GBs": d2h / 1e9,
-            "input_bytes": N1 * 32, "output_bytes": N1 * 8 + nh1[0] * 64,
-            "h2d_ms": N1 * 32 / h2d * 1e3, "d2h_ms": (N1 * 8 + nh1[0] * 64) / d2h * 1e3,
-            "note": "measured pinned copy rates; a host-buffer call adds these copies of the "
-                    "plain domain's input and outputs (ring + labels + halos) to ms_per_step"}
-    log("pcie:", pcie)
+    pcie = None
+    if not args.only_batched:
+        h2d, d2h = pcie_rates(dev)
+        pcie = {"h2d_GBs": h2d / 1e9, "d2h_GBs": d2h / 1e9,
+                "input_bytes": N1 * 32, "output_bytes": N1 * 8 + nh1[0] * 64,
+                "h2d_ms": N1 * 32 / h2d * 1e3, "d2h_ms": (N1 * 8 + nh1[0] * 64) / d2h * 1e3,
+                "note": "measured pinned copy rates; a host-buffer call adds these copies of the "
+                        "plain domain's input and outputs (ring + labels + halos) to ms_per_step"}
+        log("pcie:", pcie)
 
     del nxt, hal, DB, lab
     torch.cuda.empty_cache()
@@ -646,7 +659,7 @@ def main():
                   "(the call's kernels captured once and replayed as one HIP graph); phase times "
                   "come from one eager single-pipeline call with the tuned dense threshold "
                   "and grid",
-        "speedup_batched_vs_plain": t_plain / t_batch,
+        "speedup_batched_vs_plain": t_plain / t_batch if not args.only_batched else None,
         "groups_identical_plain_vs_batched": same_groups,
         "groups_exact_vs_oracle_on_sample": parity,
         "halo_err": {"config": "1024^3 box, halos of the parity sample vs compensated FP64 sums",

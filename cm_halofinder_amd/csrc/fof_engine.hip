@@ -139,6 +139,13 @@ __device__ __forceinline__ CellPos cell_pos(const VolDev &V, float x, float y, f
   return c;
 }
 
+// A float4 load rebuilt from its components: copies of the vector type
+// itself into a private array keep the array in scratch memory.
+__device__ __forceinline__ float4 ld4(const float4 *p) {
+  const float4 v = *p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Fine (sub-cell) coordinates of a particle of a dense cell, 21 bits per axis,
 // recorded once by k_subsort and read by the link kernels.
 __device__ __forceinline__ unsigned long long pack_fine(int fx, int fy, int fz) {
@@ -692,12 +699,13 @@ __global__ __launch_bounds__(kPartNT) void k_pb_scatter(const uint32_t *__restri
     float4 pp[kPer];
 #pragma unroll
     for (int q = 0; q < kPer; q++) {
-      const uint32_t i = threadIdx.x + q * kPartNT;
-      if (i < nsub) {
-        kk[q] = ikey[q0 + i];
-        bk[q] = (kk[q] >> kFineBits) & 255u;
-        pp[q] = ipos[q0 + i];
-      }
+      // loads clamped to the sub-tile (the extra elements are masked in
+      // staged_part), so the arrays live in registers, not scratch
+      const uint32_t i = min(threadIdx.x + q * kPartNT, nsub - 1);
+      kk[q] = ikey[q0 + i];
+      bk[q] = (kk[q] >> kFineBits) & 255u;
+      const float4 v = ipos[q0 + i];
+      pp[q] = make_float4(v.x, v.y, v.z, v.w);
     }
     staged_part<256>(nsub, kk, pp, bk, 256, cur, lcnt, loff, skey, spos, sbk, ws, okey, opos);
   }
@@ -1195,7 +1203,7 @@ __global__ __launch_bounds__(256, 8) void k_link_sparse(uint32_t N, const float4
           const uint32_t n = je - j;
           float4 q[4];
 #pragma unroll
-          for (int k = 0; k < 4; k++) q[k] = ((uint32_t)k < n) ? pos4[j + k] : px;
+          for (int k = 0; k < 4; k++) q[k] = ((uint32_t)k < n) ? ld4(pos4 + j + k) : px;
           uint32_t hit = 0;
 #pragma unroll
           for (int k = 0; k < 4; k++)
@@ -1247,7 +1255,7 @@ __global__ __launch_bounds__(256, 8) void k_link_sparse(uint32_t N, const float4
         for (; j + 4 <= je; j += 4) {
           float4 q[4];
 #pragma unroll
-          for (int k = 0; k < 4; k++) q[k] = pos4[j + k];
+          for (int k = 0; k < 4; k++) q[k] = ld4(pos4 + j + k);
           uint32_t hit = 0;
 #pragma unroll
           for (int k = 0; k < 4; k++) hit |= pair_linked(px, q[k], R, rechecks) ? (1u << k) : 0u;
@@ -1360,7 +1368,7 @@ __device__ __forceinline__ bool runs_link(const float4 *__restrict__ pos4, uint3
     for (; j + 2 <= eb; j += 2) {
       float4 q[2];
 #pragma unroll
-      for (int k = 0; k < 2; k++) q[k] = pos4[j + k];
+      for (int k = 0; k < 2; k++) q[k] = ld4(pos4 + j + k);
       bool any = false;
 #pragma unroll
       for (int k = 0; k < 2; k++) any |= pair_linked(pi, q[k], R, rechecks);
@@ -3117,7 +3125,9 @@ static std::vector<uint32_t> split_volumes(const fof_volume_v1 *vols, uint32_t n
   for (uint32_t v = 0; v < nvol; v++) {
     acc += vols[v].n;
     const uint32_t g = (uint32_t)cut.size();
-    if (g < k && acc * k >= tot * g && v + 1 < nvol && nvol - (v + 1) >= k - g) cut.push_back(v + 1);
+    // cut after v when this group has its share, or when every later group
+    // needs one of the remaining volumes
+    if (g < k && v + 1 < nvol && (acc * k >= tot * g || nvol - (v + 1) <= k - g)) cut.push_back(v + 1);
   }
   cut.push_back(nvol);
   return cut;

@@ -45,7 +45,9 @@ def test_pipelines_match_single(ctx, pipes):
     got, st = host_call(ctx, P, vols, L.opts(0, 0.0, 0, pipes))
     assert st.pipelines == min(pipes, len(vols)) and st1.pipelines == 1
     assert same(ref, got)
-    assert st.pair_tests == st1.pair_tests and st.nhalos == st1.nhalos
+    # pair-test counts vary run to run (merges found concurrently prune
+    # tests); the groups do not
+    assert st.nhalos == st1.nhalos and st.ngroups == st1.ngroups
 
 
 def test_pipelines_device_outputs_gather_halos(ctx):

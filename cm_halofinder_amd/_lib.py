@@ -19,6 +19,7 @@ ABI_VERSION = 1
 MEM_HOST, MEM_DEVICE = 0, 1
 OPT_TIMING, OPT_OUT_DEVICE, OPT_NO_RING, OPT_NO_HALOS = 0x1, 0x2, 0x4, 0x8
 OPT_GRAPH, OPT_SPARSE, OPT_AUTOTUNE = 0x10, 0x20, 0x40
+NOALLOC_BYTES = 256 << 20  # FOF_NOALLOC_BYTES
 
 E_ARG, E_DOMAIN, E_RANGE, E_NOMEM, E_HIP, E_NODEV = -1, -2, -3, -4, -5, -6
 
@@ -29,6 +30,7 @@ ABI_SYMBOLS = (
     "fof_link_gpu",
     "fof_link_gpu_batch",
     "fof_link_gpu_ex",
+    "fof_link_gpu_noalloc",
     "fof_plan",
     "fof_strerror",
     "fof_device_info",
@@ -286,6 +288,9 @@ def lib():
             vp, C.POINTER(Particles), C.POINTER(Volume), u32, C.POINTER(Opts), C.POINTER(Outputs),
         ]
         L.fof_link_gpu_ex.restype = C.c_int
+        L.fof_link_gpu_noalloc.argtypes = [vp, C.POINTER(Particles), C.POINTER(Domain),
+                                           C.POINTER(Opts), C.POINTER(Outputs)]
+        L.fof_link_gpu_noalloc.restype = C.c_int
         L.fof_plan.argtypes = [C.POINTER(Particles), C.POINTER(Volume), u32, C.POINTER(Opts),
                                C.POINTER(Plan)]
         L.fof_plan.restype = C.c_int
@@ -448,6 +453,19 @@ class Context:
         out.stats = C.cast(C.pointer(stats), C.c_void_p) if stats is not None else None
         rc = lib().fof_link_gpu_ex(self.h, C.byref(parts), arr, len(vols),
                                    C.byref(o) if o else None, C.byref(out))
+        return rc, out.nhalos
+
+    def link_noalloc(self, parts: Particles, dom: Domain, o: Opts | None = None, next_out=None,
+                     label_out=None, minid_out=None, halos=None, moments=None, halo_cap=0):
+        """fof_link_gpu_noalloc (device-resident arrays only): returns (rc, nhalos)."""
+        out = Outputs()
+        out.version = ABI_VERSION
+        out.location = MEM_DEVICE
+        out.next, out.label, out.minid = ptr(next_out)[0], ptr(label_out)[0], ptr(minid_out)[0]
+        out.halos, out.moments = ptr(halos)[0], ptr(moments)[0]
+        out.halo_cap = halo_cap
+        rc = lib().fof_link_gpu_noalloc(self.h, C.byref(parts), C.byref(dom),
+                                        C.byref(o) if o else None, C.byref(out))
         return rc, out.nhalos
 
     def link_batch(self, parts: Particles, vols, o: Opts | None = None, next_out=None,

@@ -227,6 +227,25 @@ int fof_link_gpu(fof_ctx *ctx, const fof_particles_v1 *particles, const fof_doma
   return rc;
 }
 
+int fof_link_gpu_noalloc(fof_ctx *ctx, const fof_particles_v1 *particles,
+                         const fof_domain_v1 *domain, const fof_opts_v1 *opts,
+                         fof_outputs_v1 *out) {
+  if (!out || out->version != FOF_ABI_VERSION) return FOF_E_ARG;
+  out->nhalos = 0;
+  int rc = check_particles(particles);
+  if (rc) return rc;
+  if ((rc = check_opts(opts))) return rc;
+  if (!domain) return FOF_E_ARG;
+  if (particles->location != FOF_MEM_DEVICE || out->location != FOF_MEM_DEVICE) return FOF_E_ARG;
+  if (out->minid && particles->off_id < 0) return FOF_E_ARG;
+  if ((rc = resolve_ctx(&ctx))) return rc;
+  fof_volume_v1 v;
+  v.offset = 0;
+  v.n = particles->n;
+  v.domain = *domain;
+  return fof_engine_run_noalloc(ctx, particles, &v, opts, out);
+}
+
 int fof_plan(const fof_particles_v1 *particles, const fof_volume_v1 *vols, uint32_t nvol,
              const fof_opts_v1 *opts, fof_plan_v1 *out) {
   if (!out) return FOF_E_ARG;

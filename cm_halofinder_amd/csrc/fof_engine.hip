@@ -2613,7 +2613,8 @@ struct Run {
 // the pipeline sees particles from 0) and carve its workspace.
 static int prepare_run(Run &r, Pipe &pp, const fof_particles_v1 *P, const fof_volume_v1 *vols,
                        uint32_t v0, uint32_t nv, const fof_opts_v1 *opts,
-                       const fof_outputs_v1 *out, bool stage_halos) {
+                       const fof_outputs_v1 *out, bool stage_halos, uint8_t *fixed,
+                       size_t fixed_bytes) {
   r.pp = &pp;
   r.v0 = v0;
   r.poff = vols[v0].offset;
@@ -2632,11 +2633,17 @@ static int prepare_run(Run &r, Pipe &pp, const fof_particles_v1 *P, const fof_vo
   pl.stage_halos = stage_halos && pl.want_halos;
   if (pl.N == 0) return 0;
   const size_t need = layout(pl, nullptr, r.B);
-  // a workspace far larger than this call needs (a previous call with fewer
-  // pipelines) is given back rather than kept beside the others
-  if (pp.arena.bytes > 2 * need && pp.arena.bytes - need > (1ull << 30)) release(pp.arena);
-  if ((rc = ensure(pp.arena, need))) return rc;
-  layout(pl, static_cast<uint8_t *>(pp.arena.ptr), r.B);
+  if (fixed) {
+    // the caller's fixed workspace: nothing is allocated
+    if (need > fixed_bytes || pl.sparse) return FOF_E_RANGE;
+    layout(pl, fixed, r.B);
+  } else {
+    // a workspace far larger than this call needs (a previous call with
+    // fewer pipelines) is given back rather than kept beside the others
+    if (pp.arena.bytes > 2 * need && pp.arena.bytes - need > (1ull << 30)) release(pp.arena);
+    if ((rc = ensure(pp.arena, need))) return rc;
+    layout(pl, static_cast<uint8_t *>(pp.arena.ptr), r.B);
+  }
   if (pl.host_input) {
     r.host_src = Pg.base;
     r.dbase = r.B.input;
@@ -3137,7 +3144,7 @@ static std::vector<uint32_t> split_volumes(const fof_volume_v1 *vols, uint32_t n
 // asked for and possible.
 static int run_call(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1 *vols,
                     uint32_t nvol, const fof_opts_v1 *opts, fof_outputs_v1 *out, uint32_t pipes,
-                    double *wall_ms) {
+                    double *wall_ms, uint8_t *fixed = nullptr, size_t fixed_bytes = 0) {
   EngineState &S = ctx->st;
   const uint32_t flags = opts ? opts->flags : 0u;
   fof_stats_v1 *stats = out->stats;
@@ -3152,7 +3159,7 @@ static int run_call(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1
     S.pipes.push_back(new Pipe);
     FOF_HIP(hipStreamCreate(&S.pipes.back()->stream));
   }
-  if (!S.sync_events) {
+  if (!S.sync_events) {  // created at context set-up; kept for safety
     FOF_HIP(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
     for (auto &e : S.join) FOF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     FOF_HIP(hipEventCreate(&S.t0));
@@ -3164,7 +3171,8 @@ static int run_call(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1
   const bool timing = (flags & FOF_OPT_TIMING) != 0;
   for (uint32_t g = 0; g < K; g++) {
     Pipe &pp = g == 0 ? S.main : *S.pipes[g];
-    if ((rc = prepare_run(runs[g], pp, P, vols, cut[g], cut[g + 1] - cut[g], opts, out, K > 1)))
+    if ((rc = prepare_run(runs[g], pp, P, vols, cut[g], cut[g + 1] - cut[g], opts, out, K > 1,
+                          fixed, fixed_bytes)))
       return rc;
   }
   const Plan &pl0 = runs[0].pl;
@@ -3425,10 +3433,58 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
   return run_call(ctx, P, vols, nvol, &o, out, best.pipes, nullptr);
 }
 
+// ---------------------------------------------------------------------------
+// Fixed workspace for callers that cannot allocate (fof_link_gpu_noalloc).
+// Each device holds its own copy of the array from the time the library's
+// code is loaded on it until the process ends; host code must use the
+// address hipGetSymbolAddress returns for the device, never the array's name.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = (size_t)256 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+uint8_t *g_scratch[64];  // per device, from hipGetSymbolAddress
+}  // namespace
+
 extern "C" int fof_engine_init(fof_ctx *ctx) {
-  FOF_HIP(hipStreamCreate(&ctx->st.main.stream));
+  EngineState &S = ctx->st;
+  FOF_HIP(hipStreamCreate(&S.main.stream));
+  // everything a later call may need besides workspace: the fork / join and
+  // timing events, and the fixed workspace's address on this device
+  FOF_HIP(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
+  for (auto &e : S.join) FOF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  FOF_HIP(hipEventCreate(&S.t0));
+  FOF_HIP(hipEventCreate(&S.t1));
+  S.sync_events = true;
+  if (ctx->device >= 0 && ctx->device < 64) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[ctx->device]) {
+      void *ptr = nullptr;
+      FOF_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      g_scratch[ctx->device] = static_cast<uint8_t *>(ptr);
+    }
+  }
   return 0;
 }
+
+extern "C" int fof_engine_run_noalloc(fof_ctx *ctx, const fof_particles_v1 *P,
+                                      const fof_volume_v1 *vol, const fof_opts_v1 *opts,
+                                      fof_outputs_v1 *out) {
+  if (ctx->device < 0 || ctx->device >= 64 || !g_scratch[ctx->device]) return FOF_E_NODEV;
+  if (P->location != FOF_MEM_DEVICE || out->location != FOF_MEM_DEVICE) return FOF_E_ARG;
+  if (vol->domain.varlink) return FOF_E_ARG;
+  fof_opts_v1 o;
+  memset(&o, 0, sizeof(o));
+  o.version = FOF_ABI_VERSION;
+  if (opts) o = *opts;
+  // no graph (instantiation allocates), no events, one pipeline, no tuning
+  o.flags &= ~(FOF_OPT_TIMING | FOF_OPT_GRAPH | FOF_OPT_AUTOTUNE);
+  o.pipelines = 1;
+  return run_call(ctx, P, vol, 1, &o, out, 1, nullptr, g_scratch[ctx->device], kNoAllocBytes);
+}
+
+extern "C" uint64_t fof_engine_noalloc_bytes(void) { return kNoAllocBytes; }
 
 static void free_pipe(Pipe &pp) {
   release(pp.arena);

@@ -138,5 +138,9 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
                               uint32_t nvol, const fof_opts_v1 *opts, fof_outputs_v1 *out);
 extern "C" int fof_engine_plan(const fof_particles_v1 *P, const fof_volume_v1 *vols,
                                uint32_t nvol, const fof_opts_v1 *opts, fof_plan_v1 *out);
+extern "C" int fof_engine_run_noalloc(fof_ctx *ctx, const fof_particles_v1 *P,
+                                      const fof_volume_v1 *vol, const fof_opts_v1 *opts,
+                                      fof_outputs_v1 *out);
+extern "C" uint64_t fof_engine_noalloc_bytes(void);
 extern "C" int fof_engine_init(fof_ctx *ctx);
 extern "C" void fof_engine_free(fof_ctx *ctx);

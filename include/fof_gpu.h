@@ -272,6 +272,23 @@ int fof_link_gpu_ex(fof_ctx *ctx, const fof_particles_v1 *particles,
                     const fof_volume_v1 *vols, uint32_t nvol,
                     const fof_opts_v1 *opts, fof_outputs_v1 *out);
 
+/* fof_link_gpu_ex for one domain without allocating device memory.  This
+ * entry point has no counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The call's workspace is a fixed array
+ * of FOF_NOALLOC_BYTES in the library, one per device, held from the time
+ * the library's code is loaded on the device until the process ends.
+ * Particles and every output must be device-resident; constant linking
+ * length only; timing, graphs, autotuning and pipelines are not used.  A
+ * domain whose workspace (fof_plan's workspace_bytes, plus 8 bytes per
+ * particle for minid) exceeds the array, or that needs occupied-cell
+ * addressing, returns FOF_E_RANGE.  Every call on a device shares that
+ * device's array: calls that overlap on one device race. */
+#define FOF_NOALLOC_BYTES (256ull << 20)
+int fof_link_gpu_noalloc(fof_ctx *ctx, const fof_particles_v1 *particles,
+                         const fof_domain_v1 *domain, const fof_opts_v1 *opts,
+                         fof_outputs_v1 *out);
+
 /* What a call would do, computed on the host (no device needed): the cell
  * addressing chosen, the grid sizes and the device workspace it needs. */
 typedef struct fof_plan_v1 {

@@ -285,6 +285,17 @@ __device__ __forceinline__ Rule make_rule(const VolDev &V, int ix, int iy, int i
   return r;
 }
 
+// The reference's test in FP64 (src_v2/FOF.c:345-357): dx = fl32(xi - xj),
+// distsq summed in double, linked unless distsq > linksq.
+__device__ __forceinline__ bool pair_exact(float ax, float ay, float az, float bx, float by,
+                                           float bz, double linksq) {
+  const float dx = ax - bx, dy = ay - by, dz = az - bz;
+  const double ex = dx, ey = dy, ez = dz;
+  double ds = ex * ex + ey * ey;
+  ds = ds + ez * ez;
+  return !(ds > linksq);
+}
+
 // The reference predicate with the FP32 fast path.
 __device__ __forceinline__ bool pair_linked(const float4 a, const float4 b, const Rule &r,
                                             uint32_t &nre) {
@@ -293,10 +304,7 @@ __device__ __forceinline__ bool pair_linked(const float4 a, const float4 b, cons
   if (d2 < r.lo) return true;
   if (d2 > r.hi) return false;
   nre++;
-  const double ex = dx, ey = dy, ez = dz;
-  double ds = ex * ex + ey * ey;
-  ds = ds + ez * ez;
-  return !(ds > r.linksq);
+  return pair_exact(a.x, a.y, a.z, b.x, b.y, b.z, r.linksq);
 }
 
 // Union-find on cell-sorted indices; roots are hooked larger-under-smaller,
@@ -1503,7 +1511,9 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
           if (cb != kNone) sB = G.start[cb], eB = G.start[cb + 1];
         }
         if (k >= 13 && eB - sB >= T) eB = sB;  // a dense backward cell links this pair itself
-        const bool dn = (eB - sB >= T);
+        // runs of a cell sub-sorted into runs (kBigCell particles or more),
+        // particles of any other (small dense cells are left unsorted)
+        const bool dn = (eB - sB >= max(T, kBigCell));
         nbS[k] = sB;
         nbDense[k] = dn ? 1 : 0;
         items = dn ? rtoff[sB] : eB - sB;  // runs of a dense cell, else particles
@@ -1810,6 +1820,430 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
     __syncthreads();
   }
   add_counters(ctr, NT == 64 ? kCtrTestsWave : kCtrTestsBig, tests, rechecks, joined);
+}
+
+// ---------------------------------------------------------------------------
+// 6b: small dense cells (T <= n < kBigCell particles) -- one wave per cell,
+// every pair tested in registers, components by bit-matrix closure.
+// ---------------------------------------------------------------------------
+// At about one particle per sub-cell (the usual case for cells below 256
+// particles) the sub-cell cliques of k_link_dense save few tests, while their
+// run-pair enumeration costs each lane a divergent walk per candidate.  Here
+// a wave tests 64 pairs per step with no branch and no memory access: each
+// lane holds one particle, the partners are held one per lane and broadcast
+// with readlane.  Pairs the FP32 test leaves undecided (within the band
+// around l^2) are re-decided in FP64 afterwards.  Links inside the cell form a
+// symmetric bit matrix in registers (lane i: rows i, i+64, ...; a ballot gives
+// the transposed columns), closed transitively (Warshall: for every k, the
+// rows holding k absorb row k), so each particle knows its component without
+// one union per link.  Neighbour particles within reach of the cell's box are
+// then tested 64 at a time against the cell; each joins the first particle it
+// links to, and one that links two components merges them in an LDS
+// union-find seeded with the components (the four largest components' masks
+// are kept in scalar registers, so a candidate skips whole components).
+// Which pairs belong to this kernel is the rule of k_link_dense: the cell's
+// own pairs, its 13 forward neighbours (src_v2/FOF.c:122-153) and, for a
+// constant l on a dense grid, its backward neighbours that are sparse.
+//
+// Each wave of a workgroup works on its own cells (no workgroup barriers);
+// the neighbours' particles are read as one index space over the 26 ranges,
+// two 64-particle chunks in flight, and the next cell's bounds are fetched
+// while the current one is linked.
+constexpr int kSmallQ = 128;    // neighbour candidates queued in LDS (< 64 + 64)
+constexpr int kSmallWaves = 4;  // waves per workgroup
+constexpr float kFar = 1e18f;   // position of an absent particle: links nothing
+
+struct SmallLds {
+  float4 apos[kBigCell];             // the cell's particles
+  unsigned long long cm[4][4];       // the four largest components (bit masks)
+  uint32_t lp[kBigCell];             // union-find over the cell's components
+  float4 qpos[kSmallQ];   // queued neighbour candidates
+  uint32_t qidx[kSmallQ];
+};
+
+// LDS written by some lanes of a wave and read by others afterwards: the
+// wave's LDS operations execute in order; this keeps the compiler from
+// moving them across.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Partners per test block (scalar registers: 4 per partner).
+constexpr int kBlk = 8;
+
+// The lane's particle (xi, yi, zi) against the kBlk partners p[0..kBlk-1]
+// in LDS, the same for every lane (broadcast reads, all issued before the
+// tests); partners k >= jn are absent (placed at -kFar, absent lanes sit at
+// +kFar, so nothing links them).  Returns bit k = linked under the reference's test: the FP32
+// decision outside the band; when any lane has a pair inside the band, the
+// block is re-decided pair by pair with pair_linked.  With COLS, cols[k] is
+// the lane mask of partner k's links (the transposed block).
+template <bool COLS>
+__device__ __forceinline__ uint32_t testblk(float xi, float yi, float zi,
+                                           const float4 *__restrict__ p, uint32_t jn,
+                                           const Rule &R, uint32_t &rechecks,
+                                           unsigned long long (&cols)[kBlk]) {
+  uint32_t lk = 0;
+  unsigned long long band = 0;  // lanes with an undecided pair (uniform)
+  float qx[kBlk], qy[kBlk], qz[kBlk];
+#pragma unroll
+  for (int k = 0; k < kBlk; k++) {
+    const float4 q0 = p[min((uint32_t)k, jn - 1)];
+    const bool in = (uint32_t)k < jn;
+    qx[k] = in ? q0.x : -kFar, qy[k] = in ? q0.y : -kFar, qz[k] = in ? q0.z : -kFar;
+  }
+#pragma unroll
+  for (int k = 0; k < kBlk; k++) {
+    const float dx = xi - qx[k], dy = yi - qy[k], dz = zi - qz[k];
+    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const bool l = d2 < R.lo;
+    const unsigned long long c = __ballot(l);
+    if (COLS) cols[k] = c;
+    lk |= l ? (1u << k) : 0u;
+    band |= __ballot(d2 <= R.hi) & ~c;
+  }
+  if (band) {  // rare: re-decide the block exactly
+    lk = 0;
+#pragma unroll
+    for (int k = 0; k < kBlk; k++) {
+      const float4 q = make_float4(qx[k], qy[k], qz[k], 0.f);
+      const bool l = pair_linked(make_float4(xi, yi, zi, 0.f), q, R, rechecks);
+      if (COLS) cols[k] = __ballot(l);
+      lk |= l ? (1u << k) : 0u;
+    }
+  }
+  return lk;
+}
+
+// The cell's own pairs (NW 64-particle tiles, 64 (NW - 1) < nA <= 64 NW;
+// the particles in S.apos): the link matrix, its components (breadth-first,
+// by pulling: a particle joins when its row meets the frontier), each
+// particle's component root (the lowest member) into lp, and the four
+// largest components' masks into S.cm.
+template <int NW>
+__device__ __forceinline__ void small_cell_own(SmallLds &S, uint32_t nA, const Rule &R,
+                                               uint32_t &rechecks) {
+  const float4 *const pa = S.apos;
+  const uint32_t lane = threadIdx.x & 63;
+  float xr[NW], yr[NW], zr[NW];
+#pragma unroll
+  for (int r = 0; r < NW; r++) {
+    const uint32_t i = 64 * r + lane;
+    const float4 q = pa[i < nA ? i : 0];
+    xr[r] = i < nA ? q.x : kFar, yr[r] = i < nA ? q.y : kFar, zr[r] = i < nA ? q.z : kFar;
+  }
+  unsigned long long rows[NW][NW];  // rows[r][w]: links of 64 r + lane to 64 w ..
+#pragma unroll
+  for (int r = 0; r < NW; r++)
+#pragma unroll
+    for (int w = 0; w < NW; w++) rows[r][w] = 0;
+#pragma unroll
+  for (int ta = 0; ta < NW; ta++) {
+#pragma unroll
+    for (int tb = ta; tb < NW; tb++) {
+      const uint32_t jn = min(64u, nA - 64 * tb);
+      for (uint32_t j0 = 0; j0 < jn; j0 += kBlk) {
+        unsigned long long cols[kBlk];
+        const uint32_t h = testblk<true>(xr[ta], yr[ta], zr[ta], pa + 64 * tb + j0, jn - j0, R,
+                                         rechecks, cols);
+        rows[ta][tb] |= (unsigned long long)h << j0;
+        if (tb != ta) {  // the transposed block: rows of 64 tb + j0 + k, word ta
+#pragma unroll
+          for (int k = 0; k < kBlk; k++)
+            if (lane == j0 + k) rows[tb][ta] = cols[k];
+        }
+      }
+    }
+  }
+  // particles present, and those linked to none (their own component)
+  unsigned long long unv[NW];
+#pragma unroll
+  for (int r = 0; r < NW; r++) {
+    const uint32_t n = min(64u, nA - 64 * r);
+    const unsigned long long present = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    unsigned long long other = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) other |= (w == r) ? (rows[r][w] & ~(1ull << lane)) : rows[r][w];
+    const unsigned long long iso = __ballot(other == 0) & present;
+    if ((iso >> lane) & 1ull) S.lp[64 * r + lane] = 64 * r + lane;
+    unv[r] = present & ~iso;
+  }
+  uint32_t csz[4] = {0, 0, 0, 0};
+  unsigned long long cmask[4][NW];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int w = 0; w < NW; w++) cmask[c][w] = 0;
+  for (;;) {
+    int sw = -1;
+#pragma unroll
+    for (int w = NW - 1; w >= 0; w--)
+      if (unv[w]) sw = w;
+    if (sw < 0) break;
+    unsigned long long comp[NW], F[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      comp[w] = (w == sw) ? (unv[w] & (~unv[w] + 1)) : 0ull;  // the seed: lowest unvisited
+      F[w] = comp[w];
+      unv[w] &= ~comp[w];
+    }
+    uint32_t seed = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++)
+      if (comp[w]) seed = 64 * w + (uint32_t)__builtin_ctzll(comp[w]);
+    for (;;) {
+      unsigned long long nx[NW], any = 0;
+#pragma unroll
+      for (int r = 0; r < NW; r++) {
+        unsigned long long meet = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) meet |= rows[r][w] & F[w];
+        nx[r] = __ballot(meet != 0) & unv[r];
+        any |= nx[r];
+      }
+      if (!any) break;
+#pragma unroll
+      for (int r = 0; r < NW; r++) comp[r] |= nx[r], unv[r] &= ~nx[r], F[r] = nx[r];
+    }
+    uint32_t size = 0;
+#pragma unroll
+    for (int r = 0; r < NW; r++) {
+      if ((comp[r] >> lane) & 1ull) S.lp[64 * r + lane] = seed;
+      size += (uint32_t)__popcll(comp[r]);
+    }
+    // keep the four largest (descending)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (size > csz[c]) {
+        const uint32_t ts = csz[c];
+        csz[c] = size;
+        size = ts;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+          const unsigned long long tm = cmask[c][w];
+          cmask[c][w] = comp[w];
+          comp[w] = tm;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int w = 0; w < NW; w++)
+      if (lane == 0) S.cm[c][w] = cmask[c][w];
+}
+
+// One cell of NW 64-particle tiles (64 (NW - 1) < nA <= 64 NW): its own
+// pairs, its neighbours' particles, the commit.  nxt: the wave's next cell
+// (kNone: none), whose bounds are fetched into nS, nE on the way.
+template <int NW>
+__device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA,
+                                           const VolDev &V, const CellPos &cp, const Rule &R,
+                                           uint32_t nbS, uint32_t nbN,
+                                           const float4 *__restrict__ pos4, const Grid &G,
+                                           uint32_t *__restrict__ par, uint32_t nxt,
+                                           uint32_t &nS, uint32_t &nE, uint32_t &tests,
+                                           uint32_t &rechecks) {
+  const uint32_t lane = threadIdx.x & 63;
+  volatile uint32_t *const lp = S.lp;
+  // (A, A)
+  small_cell_own<NW>(S, nA, R, rechecks);
+  tests += (lane == 0) ? nA * (nA - 1) / 2 : 0u;
+  wave_sync();
+  // the next cell's bounds (its index has arrived by now)
+  if (nxt != kNone) nS = G.start[nxt], nE = G.start[nxt + 1];
+  // (A, B): the 26 ranges as one index space: off = exclusive prefix
+  uint32_t off = nbN;
+#pragma unroll
+  for (int o = 1; o < 32; o <<= 1) {
+    const uint32_t t = __shfl_up(off, o, 64);
+    if (lane >= (uint32_t)o) off += t;
+  }
+  const uint32_t total = __shfl(off, 25, 64);
+  off -= nbN;
+  // the cell's box: a neighbour particle farther than l from it links to
+  // none of the cell's particles (prune2 carries a 1e-4 margin on l^2)
+  double lo[3], hi[3];
+  const int ic[3] = {cp.ix, cp.iy, cp.iz};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    lo[a] = V.rmin_buff[a] + ic[a] * V.lcell[a];
+    hi[a] = lo[a] + V.lcell[a];
+  }
+  // index g of the space -> particle
+  auto locate = [&](uint32_t g) -> uint32_t {
+    uint32_t base = 0;
+    for (int k = 0; k < 26; k++) {
+      const uint32_t ok = (uint32_t)__builtin_amdgcn_readlane((int)off, k);
+      const uint32_t sk = (uint32_t)__builtin_amdgcn_readlane((int)nbS, k);
+      if (g >= ok) base = sk - ok;
+    }
+    return base + g;
+  };
+  uint32_t qn = 0;  // uniform
+  auto flush = [&](uint32_t cnt) {
+    wave_sync();
+    const bool vb = lane < cnt;
+    const float4 pb = S.qpos[vb ? lane : 0];
+    const uint32_t jb = S.qidx[vb ? lane : 0];
+    const float bx = vb ? pb.x : kFar, by = vb ? pb.y : kFar, bz = vb ? pb.z : kFar;
+    tests += vb ? nA : 0u;
+    unsigned long long m[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      const uint32_t jn = min(64u, nA - 64 * w);
+      unsigned long long mw = 0;
+      for (uint32_t j0 = 0; j0 < jn; j0 += kBlk) {
+        unsigned long long cols[kBlk];
+        mw |= (unsigned long long)testblk<false>(bx, by, bz, S.apos + 64 * w + j0, jn - j0, R,
+                                                 rechecks, cols)
+              << j0;
+      }
+      m[w] = vb ? mw : 0ull;
+    }
+    // join the first linked particle; every further component linked
+    // merges with its component
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < NW; q++) any |= m[q] != 0;
+    if (any) {
+      int i0 = -1;
+#pragma unroll
+      for (int q = NW - 1; q >= 0; q--)
+        if (m[q]) i0 = 64 * q + __builtin_ctzll(m[q]);
+      uf_union(par, jb, sA + (uint32_t)i0);
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        bool hit = false;
+#pragma unroll
+        for (int q = 0; q < NW; q++) hit |= (m[q] & S.cm[c][q]) != 0;
+        if (!hit) continue;
+        int ic0 = -1;
+#pragma unroll
+        for (int q = NW - 1; q >= 0; q--)
+          if (m[q] & S.cm[c][q]) ic0 = 64 * q + __builtin_ctzll(m[q] & S.cm[c][q]);
+        lunion(lp, (uint32_t)i0, (uint32_t)ic0);
+#pragma unroll
+        for (int q = 0; q < NW; q++) m[q] &= ~S.cm[c][q];
+      }
+      // members of smaller components, one at a time
+      for (;;) {
+        int i = -1;
+#pragma unroll
+        for (int q = NW - 1; q >= 0; q--)
+          if (m[q]) i = 64 * q + __builtin_ctzll(m[q]);
+        if (i < 0) break;
+        lunion(lp, (uint32_t)i0, (uint32_t)i);
+#pragma unroll
+        for (int q = 0; q < NW; q++)
+          if (q == (i >> 6)) m[q] &= m[q] - 1;
+      }
+    }
+    wave_sync();
+  };
+  auto push = [&](float4 pb, uint32_t j, bool valid) {
+    bool keep = false;
+    if (valid) {
+      const double x = pb.x, y = pb.y, z = pb.z;
+      const double gx = fmax(fmax(lo[0] - x, x - hi[0]), 0.0);
+      const double gy = fmax(fmax(lo[1] - y, y - hi[1]), 0.0);
+      const double gz = fmax(fmax(lo[2] - z, z - hi[2]), 0.0);
+      keep = !(gx * gx + gy * gy + gz * gz > R.prune2);
+    }
+    const unsigned long long bal = __ballot(keep);
+    if (keep) {
+      const uint32_t at = qn + (uint32_t)__popcll(bal & ((1ull << lane) - 1));
+      S.qpos[at] = pb;
+      S.qidx[at] = j;
+    }
+    qn += (uint32_t)__popcll(bal);
+    if (qn >= 64) {
+      flush(64);
+      // move the overflow down (qn - 64 < 64 entries)
+      const bool mv = lane < qn - 64;
+      const float4 t = S.qpos[64 + (mv ? lane : 0)];
+      const uint32_t ti = S.qidx[64 + (mv ? lane : 0)];
+      wave_sync();
+      if (mv) S.qpos[lane] = t, S.qidx[lane] = ti;
+      qn -= 64;
+    }
+  };
+  for (uint32_t g0 = 0; g0 < total; g0 += 128) {
+    const uint32_t ga = g0 + lane, gb = g0 + 64 + lane;
+    const bool va = ga < total, vb = gb < total;
+    const uint32_t ja = va ? locate(ga) : 0u, jb = vb ? locate(gb) : 0u;
+    const float4 pa = va ? ld4(pos4 + ja) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 pb = vb ? ld4(pos4 + jb) : make_float4(0.f, 0.f, 0.f, 0.f);
+    push(pa, ja, va);
+    if (g0 + 64 < total) push(pb, jb, vb);
+  }
+  if (qn) flush(qn);
+  wave_sync();
+  // commit the cell's components (and the merges made through
+  // neighbours): a particle still its own global root is hooked under its
+  // component's root with one CAS (the root is lower: larger-under-
+  // smaller), any other joins by union
+  for (uint32_t i = lane; i < nA; i += 64) {
+    const uint32_t r = lfind(lp, i);
+    if (r != i && atomicCAS(par + sA + i, sA + i, sA + r) != sA + i) uf_union(par, sA + i, sA + r);
+  }
+  wave_sync();
+}
+
+__global__ __launch_bounds__(64 * kSmallWaves) void k_link_small(
+    const uint32_t *__restrict__ dense, const uint32_t *__restrict__ ndense,
+    const float4 *__restrict__ pos4, const Grid G, uint32_t T, uint32_t *__restrict__ par,
+    const VolDev *__restrict__ vols, const uint32_t *__restrict__ vbeg, int nvol,
+    unsigned long long *__restrict__ ctr) {
+  // the wave's index, made visibly uniform so that the cell's bounds, its
+  // particles as partners and the scan state live in scalar registers
+  const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  __shared__ SmallLds Sall[kSmallWaves];
+  SmallLds &S = Sall[wv];
+  uint32_t tests = 0, rechecks = 0;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t nd = *ndense;
+  const uint32_t w0 = blockIdx.x * kSmallWaves + wv;
+  const uint32_t ws = gridDim.x * kSmallWaves;
+  // bounds of the wave's next cell, fetched ahead
+  uint32_t nS = 0, nE = 0;
+  if (w0 < nd) {
+    const uint32_t c = dense[w0];
+    nS = G.start[c], nE = G.start[c + 1];
+  }
+  for (uint32_t d = w0; d < nd; d += ws) {
+    const uint32_t sA = nS, nA = nE - nS;  // T <= nA < kBigCell
+    const uint32_t cN = (d + ws < nd) ? dense[d + ws] : 0u;
+    const VolDev &V = vols[vol_of(vbeg, nvol, sA)];
+    const float4 p0 = pos4[sA];
+    for (uint32_t i = lane; i < nA; i += 64) S.apos[i] = ld4(pos4 + sA + i);
+    const CellPos cp = cell_pos(V, p0.x, p0.y, p0.z);
+    const Rule R = make_rule(V, cp.ix, cp.iy, cp.iz);
+    // neighbour k of the table (lane k): its particle range
+    uint32_t nbS = 0, nbN = 0;
+    if (lane < 26) {
+      int dd[3];
+      nb_offset((int)lane, dd);
+      const int bx = cp.ix + dd[0], by = cp.iy + dd[1], bz = cp.iz + dd[2];
+      const bool want = lane < 13 || (!G.sparse && V.vntab == 0);
+      if (want && !(bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz < 0 || bz >= V.nz)) {
+        const uint32_t cb = cell_index(G, V, bx, by, bz);
+        if (cb != kNone) nbS = G.start[cb], nbN = G.start[cb + 1] - nbS;
+      }
+      if (lane >= 13 && nbN >= T) nbN = 0;  // a dense backward cell links this pair itself
+    }
+    wave_sync();
+    const uint32_t nw = (nA + 63) >> 6;
+    const uint32_t nxt = (d + ws < nd) ? cN : kNone;
+    if (nw <= 1) small_cell<1>(S, sA, nA, V, cp, R, nbS, nbN, pos4, G, par, nxt, nS, nE, tests, rechecks);
+    else if (nw == 2) small_cell<2>(S, sA, nA, V, cp, R, nbS, nbN, pos4, G, par, nxt, nS, nE, tests, rechecks);
+    else if (nw == 3) small_cell<3>(S, sA, nA, V, cp, R, nbS, nbN, pos4, G, par, nxt, nS, nE, tests, rechecks);
+    else small_cell<4>(S, sA, nA, V, cp, R, nbS, nbN, pos4, G, par, nxt, nS, nE, tests, rechecks);
+  }
+  add_counters(ctr, kCtrTestsWave, tests, rechecks);
 }
 
 // ---------------------------------------------------------------------------
@@ -2855,10 +3289,6 @@ static int enqueue_run(Run &r, hipStream_t st) {
   for (uint32_t v = 0; v < nvol; v++) any_sub = any_sub || pl.hv[v].d.m[0] > 0;
   if (any_sub) {
     if ((rc = tb(4))) return rc;
-    k_subsort<64, true><<<gsml, 64, 0, st>>>(dense, small + kSmNDense, start, dV, dVB, (int)nvol,
-                                             pos4, reinterpret_cast<float4 *>(sk0), par, runend,
-                                             fine, rtoff, rtab);
-    FOF_HIP(hipGetLastError());
     k_subsort<256, false><<<gbig, 256, 0, st>>>(dbig, small + kSmNBig, start, dV, dVB, (int)nvol,
                                                 pos4, reinterpret_cast<float4 *>(sk0), par, runend,
                                                 fine, rtoff, rtab);
@@ -2891,12 +3321,8 @@ static int enqueue_run(Run &r, hipStream_t st) {
     FOF_HIP(hipGetLastError());
     if ((rc = te(6))) return rc;
     if ((rc = tb(15))) return rc;
-    constexpr int kRBw = 64;
-    const uint32_t raw = std::min(maxbins, kBigCell - 1);
-    k_link_dense<64, 6, (int)kBigCell - 1, kRBw, 64, 256, 8>
-        <<<gsml, 64, dense_lds(raw, kRBw, true), st>>>(
-            dense, small + kSmNDense, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB,
-            (int)nvol, ctr, small + kSmOverflow, raw);
+    k_link_small<<<(gsml + kSmallWaves - 1) / kSmallWaves, 64 * kSmallWaves, 0, st>>>(
+        dense, small + kSmNDense, pos4, G, T, par, dV, dVB, (int)nvol, ctr);
     FOF_HIP(hipGetLastError());
     if ((rc = te(15))) return rc;
   }

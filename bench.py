@@ -90,15 +90,15 @@ def candidate_pairs(recs_u8, vols, T):
     (src_v2/FOF.c:94-161: every pair inside a cell plus every pair with the
     13 forward neighbours, on the reference grid of :45-65), per volume,
     charged to the kernel that tests them.  Dense cells (at least T
-    particles, sub-cells enabled) belong to k_link_dense (256-thread
-    workgroups from BIG_CELL particles, one wave below); for a constant
+    particles, sub-cells enabled) belong to k_link_dense from BIG_CELL
+    particles and to k_link_small below; for a constant
     linking length a pair touching a dense cell is tested by that cell's
     kernel (the lower cell's when both are dense), otherwise by the home
-    cell's.  Returns ({"big": .., "wave": .., "sparse": ..}, dense_cells)."""
+    cell's.  Returns ({"big": .., "small": .., "sparse": ..}, dense_cells)."""
     import torch
 
     xyz = recs_u8.view(torch.float32).view(-1, 8)[:, 2:5]
-    tot = {"big": 0, "wave": 0, "sparse": 0}
+    tot = {"big": 0, "small": 0, "sparse": 0}
     nd = 0
     for v in vols:
         d = v.domain
@@ -119,7 +119,7 @@ def candidate_pairs(recs_u8, vols, T):
             idx[c0:c1] = (t[:, 2] * dims[1] + t[:, 1]) * dims[0] + t[:, 0]
         cnt = torch.bincount(idx, minlength=dims[0] * dims[1] * dims[2]).view(dims[2], dims[1], dims[0])
         del idx
-        # class of every cell: 0 sparse, 1 wave, 2 big
+        # class of every cell: 0 sparse, 1 small dense, 2 big dense
         cls = torch.zeros_like(cnt, dtype=torch.int8)
         if sub_on:
             cls[cnt >= T] = 1
@@ -144,7 +144,7 @@ def candidate_pairs(recs_u8, vols, T):
             klass = torch.where(ka > 0, ka, kb) if by_dense else ka
             add(ca * cb, klass)
         tot["sparse"] += int(acc[0].item())
-        tot["wave"] += int(acc[1].item())
+        tot["small"] += int(acc[1].item())
         tot["big"] += int(acc[2].item())
         del cnt, cls
     return tot, nd
@@ -193,15 +193,15 @@ def roofline(stats, L, recs_u8, vols, N, cells, workload=None):
     ph = dict(zip(L.PHASES, stats.phase_ms))
     tests = dict(zip(L.LINK_KERNELS, [int(x) for x in stats.pair_tests_kernel]))
     phase_of = {"k_link_sparse": "link_sparse", "k_link_dense": "link_dense",
-                "k_link_dense_wave": "link_dense_wave"}
+                "k_link_small": "link_small"}
     kern = max(phase_of, key=lambda k: ph[phase_of[k]])
     ms = ph[phase_of[kern]]
     tf = tests[kern] * 8 / (ms * 1e-3) / 1e12
     cand, nd = candidate_pairs(recs_u8, vols, stats.dense_threshold)
-    cand_by = {"k_link_dense": cand["big"], "k_link_dense_wave": cand["wave"],
+    cand_by = {"k_link_dense": cand["big"], "k_link_small": cand["small"],
                "k_link_sparse": cand["sparse"]}
     prof, src = hbm_profile(workload) if workload else ({}, None)
-    link_ms = ph["link_sparse"] + ph["link_dense"] + ph["link_dense_wave"]
+    link_ms = ph["link_sparse"] + ph["link_dense"] + ph["link_small"]
     total_tests = int(stats.pair_tests)
     out = {
         "kernel": kern,

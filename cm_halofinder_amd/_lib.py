@@ -115,7 +115,7 @@ assert HALO_DTYPE.itemsize == C.sizeof(Halo) == 64
 NPHASE = 17
 PHASES = ("keys", "scan", "scatter", "uf_init", "subsort", "link_sparse", "link_dense",
           "flatten", "label", "compact", "ring_sort", "ring", "halo_list", "halo_sums", "output",
-          "link_dense_wave", "total")
+          "link_small", "total")
 
 
 class Stats(C.Structure):
@@ -152,7 +152,7 @@ class Stats(C.Structure):
 
 
 # pair_tests_kernel order
-LINK_KERNELS = ("k_link_sparse", "k_link_dense", "k_link_dense_wave")
+LINK_KERNELS = ("k_link_sparse", "k_link_dense", "k_link_small")
 
 
 class Moments(C.Structure):

@@ -14,13 +14,17 @@
 //                    counting sort per fine bucket -> cell starts, dense-cell
 //                    lists and float4 {x,y,z,index} in cell order
 //                    (sparse addressing: composite keys, radix sort, hash)
-//   4 k_subsort      cells with >= T particles: counting sort by sub-cell;
-//                    sub-cells have side <= l/sqrt(3), so each is a clique and
-//                    its particles start in one union-find set ("runs")
+//   4 k_subsort      cells with >= max(T, 256) particles: counting sort by
+//                    sub-cell; sub-cells have side <= l/sqrt(3), so each is a
+//                    clique and its particles start in one union-find set
+//                    ("runs")
 //   5 k_link_sparse  one thread per particle of a sparse cell: own cell and
 //                    the 13 forward cells (src_v2/FOF.c:112-153)
-//   6 k_link_dense   dense cells, one workgroup (256 particles or more) or
-//                    one wave each: run pairs with early exit
+//   6 k_link_dense   dense cells of 256 particles or more, one workgroup
+//                    each: run pairs with early exit
+//     k_link_small   dense cells below 256 particles, one wave each: every
+//                    pair tested from LDS broadcasts, components by BFS over
+//                    a bit matrix in registers
 //   7 union-find finish: flatten, group minimum index, sizes
 //   8 labels (members' pairs through scatter_u32), compaction of kept
 //     groups, stable radix sort by label
@@ -393,7 +397,7 @@ constexpr int kCtrRechecks = 1;     // tests re-decided in FP64 (all kernels)
 constexpr int kCtrGroups = 2;       // groups of any size
 constexpr int kCtrOwned = 3;        // owned halos
 constexpr int kCtrTestsBig = 4;     // pair tests in k_link_dense, 256 threads
-constexpr int kCtrTestsWave = 5;    // pair tests in k_link_dense, one wave
+constexpr int kCtrTestsSmall = 5;   // pair tests in k_link_small
 constexpr int kCtrJoined = 6;       // candidate run pairs found already joined
 constexpr int kCtrSlots = 64;
 constexpr int kCtrStride = 16;  // u64 per slot
@@ -994,14 +998,12 @@ __device__ __forceinline__ int sub_bin(const VolDev &V, const float4 p) {
   return (sub1(uz, iz, V.m[2]) * V.m[1] + sub1(uy, iy, V.m[1])) * V.m[0] + sub1(ux, ix, V.m[0]);
 }
 
-// Sub-cell counting sort, one workgroup of NT threads per dense cell.  With
-// SMALL (cells below 256 particles, one wave) every count fits a byte:
-// counters are packed four to a word and starts are bytes, so the LDS
-// footprint stays near 4 KB; the particles' sub-cells are kept in LDS
-// between the counting and the scatter pass.
+// Sub-cell counting sort, one workgroup of NT threads per dense cell of the
+// list (cells of 256 particles or more; smaller dense cells stay unsorted:
+// k_link_small tests all their pairs).
 constexpr int kMaxCellBins = 12 * 12 * 12;  // m <= 12 per axis
 
-template <int NT, bool SMALL>
+template <int NT>
 __global__ __launch_bounds__(NT) void k_subsort(const uint32_t *__restrict__ dense,
                                                 const uint32_t *__restrict__ ndense,
                                                 const uint32_t *__restrict__ start,
@@ -1015,28 +1017,11 @@ __global__ __launch_bounds__(NT) void k_subsort(const uint32_t *__restrict__ den
                                                 uint32_t *__restrict__ rtoff,
                                                 uint32_t *__restrict__ rtab) {
   constexpr int kWaves = NT / 64;
-  // bins per thread in the scans (a multiple of 4 when counters are packed)
-  constexpr int kPer = SMALL ? ((kMaxCellBins + 4 * NT - 1) / (4 * NT)) * 4
-                             : (kMaxCellBins + NT - 1) / NT;
-  constexpr int kWords = SMALL ? kPer * NT / 4 : kMaxCellBins;
-  __shared__ uint32_t bcur[kWords];                          // cursors (packed when SMALL)
-  __shared__ uint32_t bstart32[SMALL ? 1 : kMaxCellBins];    // run starts
-  __shared__ uint8_t bstart8[SMALL ? kPer * NT : 1];
-  __shared__ uint16_t pbin[SMALL ? 256 : 1];
+  constexpr int kPer = (kMaxCellBins + NT - 1) / NT;  // bins per thread in the scans
+  __shared__ uint32_t bcur[kMaxCellBins];    // cursors
+  __shared__ uint32_t bstart[kMaxCellBins];  // run starts
   __shared__ uint32_t ws[kWaves];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  auto get_cur = [&](int b) -> uint32_t {
-    return SMALL ? (bcur[b >> 2] >> ((b & 3) * 8)) & 0xFFu : bcur[b];
-  };
-  auto get_start = [&](int b) -> uint32_t { return SMALL ? bstart8[b] : bstart32[b]; };
-  // fetch-and-increment of a bin's counter
-  auto bump = [&](int b) -> uint32_t {
-    if (SMALL) {
-      const int sh = (b & 3) * 8;
-      return (atomicAdd(&bcur[b >> 2], 1u << sh) >> sh) & 0xFFu;
-    }
-    return atomicAdd(&bcur[b], 1u);
-  };
   // block-wide exclusive scan of one value per thread; returns (prefix, total)
   auto block_scan = [&](uint32_t loc, uint32_t &tot) -> uint32_t {
     uint32_t inc = loc;
@@ -1044,10 +1029,6 @@ __global__ __launch_bounds__(NT) void k_subsort(const uint32_t *__restrict__ den
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t t = __shfl_up(inc, o, 64);
       if (lane >= o) inc += t;
-    }
-    if (kWaves == 1) {
-      tot = __shfl(inc, 63, 64);
-      return inc - loc;
     }
     if (lane == 63) ws[w] = inc;
     __syncthreads();
@@ -1063,53 +1044,32 @@ __global__ __launch_bounds__(NT) void k_subsort(const uint32_t *__restrict__ den
   const uint32_t nd = *ndense;
   for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
     const uint32_t c = dense[d];
-    const uint32_t s = start[c], e = start[c + 1], n = e - s;  // SMALL: n < 256
+    const uint32_t s = start[c], e = start[c + 1], n = e - s;
     const VolDev &V = vols[vol_of(vbeg, nvol, s)];
     const int nbins = V.m[0] * V.m[1] * V.m[2];
-    for (int b = threadIdx.x; b < kWords; b += NT) bcur[b] = 0;
+    for (int b = threadIdx.x; b < kMaxCellBins; b += NT) bcur[b] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += NT) {
-      const int b = sub_bin(V, pos4[s + i]);
-      if (SMALL) pbin[i] = (uint16_t)b;
-      bump(b);
-    }
+    for (uint32_t i = threadIdx.x; i < n; i += NT) atomicAdd(&bcur[sub_bin(V, pos4[s + i])], 1u);
     __syncthreads();
     // exclusive scan of the counts -> starts and cursors (kPer bins a thread)
     {
       const int b0 = threadIdx.x * kPer;
       uint32_t loc = 0, tot;
       for (int q = 0; q < kPer; q++)
-        if (b0 + q < nbins) loc += get_cur(b0 + q);
+        if (b0 + q < nbins) loc += bcur[b0 + q];
       uint32_t pre = block_scan(loc, tot);
-      if (SMALL) {
-        for (int q = 0; q < kPer; q += 4) {  // whole words: no other thread shares them
-          uint32_t word = 0;
-          for (int k = 0; k < 4; k++) {
-            const int b = b0 + q + k;
-            if (b < nbins) {
-              const uint32_t cc = get_cur(b);
-              bstart8[b] = (uint8_t)pre;
-              word |= pre << (8 * k);
-              pre += cc;
-            }
-          }
-          bcur[(b0 + q) >> 2] = word;
+      for (int q = 0; q < kPer; q++)
+        if (b0 + q < nbins) {
+          const uint32_t cc = bcur[b0 + q];
+          bstart[b0 + q] = pre;
+          bcur[b0 + q] = pre;
+          pre += cc;
         }
-      } else {
-        for (int q = 0; q < kPer; q++)
-          if (b0 + q < nbins) {
-            const uint32_t cc = bcur[b0 + q];
-            bstart32[b0 + q] = pre;
-            bcur[b0 + q] = pre;
-            pre += cc;
-          }
-      }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += NT) {
       const float4 p = pos4[s + i];
-      const uint32_t slot = bump(SMALL ? (int)pbin[i] : sub_bin(V, p));
-      tmp4[s + slot] = p;
+      tmp4[s + atomicAdd(&bcur[sub_bin(V, p)], 1u)] = p;
     }
     __syncthreads();
     // run table of the cell: heads in sub-cell order at rtab[s ..), their
@@ -1118,12 +1078,12 @@ __global__ __launch_bounds__(NT) void k_subsort(const uint32_t *__restrict__ den
       const int b0 = threadIdx.x * kPer;
       uint32_t loc = 0, tot;
       for (int q = 0; q < kPer; q++)
-        if (b0 + q < nbins && get_cur(b0 + q) != get_start(b0 + q)) loc++;
+        if (b0 + q < nbins && bcur[b0 + q] != bstart[b0 + q]) loc++;
       const uint32_t pre = block_scan(loc, tot);
       if (threadIdx.x == 0) rtoff[s] = tot;
       uint32_t r = s + pre;
       for (int q = 0; q < kPer; q++)
-        if (b0 + q < nbins && get_cur(b0 + q) != get_start(b0 + q)) rtab[r++] = s + get_start(b0 + q);
+        if (b0 + q < nbins && bcur[b0 + q] != bstart[b0 + q]) rtab[r++] = s + bstart[b0 + q];
     }
     for (uint32_t i = threadIdx.x; i < n; i += NT) {
       const float4 p = tmp4[s + i];
@@ -1131,8 +1091,8 @@ __global__ __launch_bounds__(NT) void k_subsort(const uint32_t *__restrict__ den
       const int b = ((cp.fz - cp.iz * V.m[2]) * V.m[1] + (cp.fy - cp.iy * V.m[1])) * V.m[0] +
                     (cp.fx - cp.ix * V.m[0]);
       pos4[s + i] = p;
-      par[s + i] = s + get_start(b);
-      run_end[s + i] = s + get_cur(b);
+      par[s + i] = s + bstart[b];
+      run_end[s + i] = s + bcur[b];
       fine[s + i] = pack_fine(cp.fx, cp.fy, cp.fz);
     }
     __syncthreads();
@@ -1425,9 +1385,9 @@ __device__ __forceinline__ void nb_offset(int k, int dd[3]) {
 }
 
 // Bytes of dynamic LDS for ra home-run slots and MAXRUNSB neighbour slots:
-// ahead, agr, lp (ra + MAXRUNSB), arel (u16), asz (u8, one-wave kernel only).
-constexpr size_t dense_lds(uint32_t ra, uint32_t maxrunsb, bool sizes) {
-  return 4 * ((size_t)3 * ra + maxrunsb) + 2 * (size_t)ra + (sizes ? (size_t)ra : 0) + 8;
+// ahead, agr, lp (ra + MAXRUNSB), arel (u16).
+constexpr size_t dense_lds(uint32_t ra, uint32_t maxrunsb) {
+  return 4 * ((size_t)3 * ra + maxrunsb) + 2 * (size_t)ra + 8;
 }
 
 // WPE: waves per SIMD to fit (the register budget; LDS per workgroup is
@@ -1454,7 +1414,6 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
   extern __shared__ uint32_t dyn[];
   uint32_t *const ahead = dyn, *const agr = dyn + ra, *const lp = dyn + 2 * ra;
   uint16_t *const arel = reinterpret_cast<uint16_t *>(dyn + 3 * ra + MAXRUNSB);
-  uint8_t *const asz = reinterpret_cast<uint8_t *>(arel + ra);
   __shared__ uint32_t bhead[MAXRUNSB], brel[MAXRUNSB], bgr[MAXRUNSB];
   __shared__ uint32_t bigq[BIGQ];
   // occupied sub-cells of A (<= 12^3) and the occupied count before each
@@ -1467,8 +1426,6 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
   __shared__ uint8_t nbDense[26];
   __shared__ uint32_t cnt, qn;
   // one-wave cells: run sizes in LDS (home runs fit a byte: < 256 particles)
-  constexpr bool SIZES = (NT == 64);
-  __shared__ uint16_t bsz[SIZES ? MAXRUNSB : 1];
   uint32_t tests = 0, rechecks = 0, joined = 0;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nd = *ndense;
@@ -1539,7 +1496,6 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
       const int rx = fine_x(f) - ox, ry = fine_y(f) - oy, rz = fine_z(f) - oz;
       ahead[k] = j;
       arel[k] = pack_home(rx, ry, rz);
-      if (SIZES) asz[k] = (uint8_t)(run_end[j] - j);
       agr[k] = uf_find(par, j);  // snapshot of the global root
       lp[k] = k;
       const int bin = (rz * mm[1] + ry) * mm[0] + rx;
@@ -1565,20 +1521,13 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
       return (uint32_t)(v & ((1ull << len) - 1));
     };
     const bool bmap = na > kMapRuns;
-    constexpr uint32_t kWalks = NT > 64 ? 2 : 1;
+    constexpr uint32_t kWalks = 2;
     // bits of x in [c - 1, c + 1] for a row starting at x0, given s = c - 1 - x0
     auto touchbits = [](int s) -> uint32_t { return s >= 0 ? 7u << s : 7u >> -s; };
     // a run pair (LDS slots sa, sb with heads ha, hb) not yet joined: small
     // pairs are tested here, big ones go to the wave queue
     auto test_pair = [&](uint32_t sa, uint32_t sb, uint32_t ha, uint32_t hb) {
-      uint32_t ea, eb;
-      if (SIZES) {
-        const uint32_t zb = sb < (uint32_t)na ? (uint32_t)asz[sb] : (uint32_t)bsz[sb - na];
-        ea = ha + asz[sa];
-        eb = (zb == 65535u) ? run_end[hb] : hb + zb;
-      } else {
-        ea = run_end[ha], eb = run_end[hb];
-      }
+      const uint32_t ea = run_end[ha], eb = run_end[hb];
       if ((ea - ha) * (eb - hb) > BIGPAIR) {
         const uint32_t q = atomicAdd(&qn, 1u);
         if (q < (uint32_t)BIGQ) {
@@ -1635,20 +1584,6 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
     // else test
     auto consider = [&](uint32_t sa, uint32_t sb, uint32_t ga, uint32_t gb, uint32_t ha,
                         uint32_t hb) {
-      if (SIZES) {  // two single particles: one test is cheaper than the LDS walks
-        const uint32_t zb = sb < (uint32_t)na ? (uint32_t)asz[sb] : (uint32_t)bsz[sb - na];
-        if (asz[sa] == 1 && zb == 1) {
-          if (ga != gb) {
-            const float4 a = pos4[ha], b = pos4[hb];
-            tests++;
-            if (!pair_linked(a, b, R, rechecks)) return;
-          } else {
-            joined++;
-          }
-          lunion(lp, sa, sb);
-          return;
-        }
-      }
       if (lfind(lp, sa) == lfind(lp, sb)) {
         joined++;
         return;
@@ -1751,7 +1686,6 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
         const uint32_t k = atomicAdd(&cnt, 1u);  // < MAXRUNSB: one per item at most
         bhead[k] = j;
         brel[k] = pack_rel(rx, ry, rz);
-        if (SIZES) bsz[k] = nbDense[q] ? (uint16_t)min(run_end[j] - j, 65535u) : (uint16_t)1;
         bgr[k] = uf_find(par, j);
         lp[na + k] = na + k;
       }
@@ -1819,7 +1753,7 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
     }
     __syncthreads();
   }
-  add_counters(ctr, NT == 64 ? kCtrTestsWave : kCtrTestsBig, tests, rechecks, joined);
+  add_counters(ctr, kCtrTestsBig, tests, rechecks, joined);
 }
 
 // ---------------------------------------------------------------------------
@@ -2243,7 +2177,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_link_small(
     else if (nw == 3) small_cell<3>(S, sA, nA, V, cp, R, nbS, nbN, pos4, G, par, nxt, nS, nE, tests, rechecks);
     else small_cell<4>(S, sA, nA, V, cp, R, nbS, nbN, pos4, G, par, nxt, nS, nE, tests, rechecks);
   }
-  add_counters(ctr, kCtrTestsWave, tests, rechecks);
+  add_counters(ctr, kCtrTestsSmall, tests, rechecks);
 }
 
 // ---------------------------------------------------------------------------
@@ -3289,7 +3223,7 @@ static int enqueue_run(Run &r, hipStream_t st) {
   for (uint32_t v = 0; v < nvol; v++) any_sub = any_sub || pl.hv[v].d.m[0] > 0;
   if (any_sub) {
     if ((rc = tb(4))) return rc;
-    k_subsort<256, false><<<gbig, 256, 0, st>>>(dbig, small + kSmNBig, start, dV, dVB, (int)nvol,
+    k_subsort<256><<<gbig, 256, 0, st>>>(dbig, small + kSmNBig, start, dV, dVB, (int)nvol,
                                                 pos4, reinterpret_cast<float4 *>(sk0), par, runend,
                                                 fine, rtoff, rtab);
     FOF_HIP(hipGetLastError());
@@ -3315,7 +3249,7 @@ static int enqueue_run(Run &r, hipStream_t st) {
     constexpr int kRB = 448;
     const uint32_t ra = std::min(maxbins, (uint32_t)kMaxRuns);
     k_link_dense<256, 8, kMaxRuns, kRB, 512, 1024, 1>
-        <<<gbig, 256, dense_lds(ra, kRB, false), st>>>(
+        <<<gbig, 256, dense_lds(ra, kRB), st>>>(
             dbig, small + kSmNBig, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol,
             ctr, small + kSmOverflow, ra);
     FOF_HIP(hipGetLastError());
@@ -3772,8 +3706,8 @@ static int run_call(fof_ctx *ctx, const fof_particles_v1 *P, const fof_volume_v1
     }
     stats->pair_tests_kernel[0] = hc[kCtrTestsSparse];
     stats->pair_tests_kernel[1] = hc[kCtrTestsBig];
-    stats->pair_tests_kernel[2] = hc[kCtrTestsWave];
-    stats->pair_tests = hc[kCtrTestsSparse] + hc[kCtrTestsBig] + hc[kCtrTestsWave];
+    stats->pair_tests_kernel[2] = hc[kCtrTestsSmall];
+    stats->pair_tests = hc[kCtrTestsSparse] + hc[kCtrTestsBig] + hc[kCtrTestsSmall];
     stats->pairs_joined = hc[kCtrJoined];
     stats->rechecks = hc[kCtrRechecks];
     stats->ngroups = hc[kCtrGroups];

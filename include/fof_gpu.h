@@ -155,6 +155,7 @@ typedef struct fof_halo_v1 {
  *   0 cell keys                     1 bucket offsets (scan) or key sort (sparse)
  *   2 partitions + bucket sort      3 union-find init
  *   4 sub-cell sort of dense cells  5 pair tests, sparse cells
+ *     of 256 particles or more
  *   6 pair tests, dense cells of    7 union-find flatten + group minima
  *     256 particles or more
  *   8 labels                        9 compaction of kept groups
@@ -183,8 +184,9 @@ typedef struct fof_stats_v1 {
   uint64_t pair_tests_kernel[3]; /* pair tests by kernel: sparse cells, dense
                                     cells of 256 particles or more, smaller
                                     dense cells (sum = pair_tests)            */
-  uint64_t pairs_joined;  /* candidate run pairs not tested because both ends
-                             were already in one set                         */
+  uint64_t pairs_joined;  /* candidate run pairs (cells of 256 particles or
+                             more) not tested because both ends were already
+                             in one set                                      */
   uint32_t pipelines;     /* concurrent pipelines the volumes were split over */
   uint32_t graph;         /* 1: replayed a captured graph, 2: captured one    */
   uint32_t big_grid;      /* workgroups of the 256-thread dense kernels       */

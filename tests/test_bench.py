@@ -18,9 +18,9 @@ def test_candidate_pairs_match_reference_sweep():
     recs = torch.from_numpy(P.view(np.uint8).reshape(-1).copy())
     for T in (1, 12, 96, 1 << 30):
         by, nd = bench.candidate_pairs(recs, vols, T)
-        assert by["big"] + by["wave"] + by["sparse"] == pairs
+        assert by["big"] + by["small"] + by["sparse"] == pairs
         if T == 1 << 30:
-            assert by["big"] == by["wave"] == 0 and nd == 0
+            assert by["big"] == by["small"] == 0 and nd == 0
     by1, nd1 = bench.candidate_pairs(recs, vols, 1)
     assert nd1 > 0 and by1["sparse"] == 0  # every occupied cell is dense at T = 1
     # a pair touching a dense cell is charged to a dense kernel

@@ -1422,6 +1422,12 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
   constexpr int kOccW = kMaxRuns / 64 + 1;
   __shared__ unsigned long long occ[kOccW];
   __shared__ uint16_t wpre[kOccW];
+  // after the touching pairs: the dominant LDS set (mainr, kNone if none)
+  // and a bitmap of the occupied sub-cells whose runs were outside it
+  // ("stragglers"); a pair of two runs of the dominant set is joined, so a
+  // run or neighbour item in it needs only the stragglers of a window row
+  __shared__ unsigned long long smap[kOccW];
+  __shared__ uint32_t mainr;
   __shared__ uint32_t nbS[26], nbOff[27];
   __shared__ uint8_t nbDense[26];
   __shared__ uint32_t cnt, qn;
@@ -1483,8 +1489,8 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
       if (k < 26) nbOff[k + 1] = items;
       if (k == 0) nbOff[0] = 0;
     }
-    if (threadIdx.x == 0) qn = 0;
-    if (threadIdx.x < kOccW) occ[threadIdx.x] = 0;
+    if (threadIdx.x == 0) qn = 0, mainr = kNone;
+    if (threadIdx.x < kOccW) occ[threadIdx.x] = 0, smap[threadIdx.x] = 0;
     __syncthreads();
     const int wy = 2 * reach[1] + 1;
     // the home cell's runs (one per occupied sub-cell, in sub-cell order)
@@ -1518,6 +1524,13 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
       const int w = bin0 >> 6, sh = bin0 & 63;
       unsigned long long v = occ[w] >> sh;
       if (sh + len > 64) v |= occ[w + 1] << (64 - sh);
+      return (uint32_t)(v & ((1ull << len) - 1));
+    };
+    // the same for the stragglers
+    auto srowbits = [&](int bin0, int len) -> uint32_t {
+      const int w = bin0 >> 6, sh = bin0 & 63;
+      unsigned long long v = smap[w] >> sh;
+      if (sh + len > 64) v |= smap[w + 1] << (64 - sh);
       return (uint32_t)(v & ((1ull << len) - 1));
     };
     const bool bmap = na > kMapRuns;
@@ -1577,7 +1590,10 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
         if (linked && gl == 0) lunion(lp, sa, sb);
       }
       __syncthreads();
-      if (threadIdx.x == 0) qn = 0;
+      if (threadIdx.x == 0) {
+        qn = 0;
+        if (mainr != kNone) mainr = lfind(lp, mainr);  // merges may hook it
+      }
       __syncthreads();
     };
     // a candidate pair of slots: skip if joined, join on equal snapshots,
@@ -1598,11 +1614,14 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
     // (A, A), in rounds each followed by a drain, so that merges found early
     // prune the later rounds (a full queue falls back to testing in place)
     if (bmap) {
-      // items (row of the forward window, run), row-major, rows centre-out
+      // items (row of the forward window, run), row-major, rows centre-out;
+      // the first walk (touching sub-cells) before the second
       const uint32_t nrow = (uint32_t)((reach[2] + 1) * wy);
-      const uint32_t total = kWalks * nrow * (uint32_t)na;
-      for (uint32_t base = 0; base < total; base += ROUND) {
-        const uint32_t pe = min(total, base + (uint32_t)ROUND);
+      const uint32_t half = nrow * (uint32_t)na;
+      auto aa_items = [&](uint32_t from, uint32_t to) {
+      for (uint32_t base = from; base < to; base += ROUND) {
+        const uint32_t pe = min(to, base + (uint32_t)ROUND);
+        const bool masked = mainr != kNone;
         for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
           const uint32_t r2 = p / (uint32_t)na, ia = p - r2 * (uint32_t)na;
           const bool first = kWalks == 2 && r2 < nrow;  // touching sub-cells only
@@ -1622,6 +1641,10 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
           const uint32_t all = rowbits(bin0, x1 - x0 + 1), r0 = rank(bin0);
           uint32_t bits = all;
           if (trow) bits &= first ? touchbits(ax - 1 - x0) : ~touchbits(ax - 1 - x0);
+          if (masked) {  // a run of the dominant set: only the stragglers
+            const int abin = (az * mm[1] + ay) * mm[0] + ax;
+            if (!((smap[abin >> 6] >> (abin & 63)) & 1ull)) bits &= srowbits(bin0, x1 - x0 + 1);
+          }
           while (bits) {
             const int b = __builtin_ctz(bits);
             bits &= bits - 1;
@@ -1633,8 +1656,36 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
         }
         drain();
       }
+      };
+      aa_items(0, half);
+      // the dominant set: a vote of 64 runs spread over the cell (wave 0);
+      // kept when it holds at least a quarter of them
+      if (threadIdx.x < 64) {
+        const uint32_t r = lfind(lp, (uint32_t)(((uint64_t)threadIdx.x * (uint32_t)na) >> 6));
+        uint32_t best = kNone, bestc = 0;
+        unsigned long long pend = __ballot(true);
+        while (pend) {
+          const uint32_t rl = __shfl(r, __ffsll((long long)pend) - 1, 64);
+          const unsigned long long m = __ballot(r == rl);
+          const uint32_t cnt1 = (uint32_t)__popcll(m);
+          if (cnt1 > bestc) bestc = cnt1, best = rl;
+          pend &= ~m;
+        }
+        if (threadIdx.x == 0) mainr = bestc >= 16 ? best : kNone;
+      }
+      __syncthreads();
+      if (mainr != kNone) {
+        for (uint32_t k = threadIdx.x; k < (uint32_t)na; k += blockDim.x)
+          if (lfind(lp, k) != mainr) {
+            const uint32_t rk = arel[k];
+            const int bin = ((rk >> 8) * mm[1] + ((rk >> 4) & 15)) * mm[0] + (rk & 15);
+            atomicOr(&smap[bin >> 6], 1ull << (bin & 63));
+          }
+      }
+      __syncthreads();
+      aa_items(half, 2 * half);
     } else {
-      const uint32_t total = (uint32_t)na * (uint32_t)na;
+      const uint32_t total = (uint32_t)na * (uint32_t)na;  // na <= kMapRuns
       for (uint32_t base = 0; base < total; base += ROUND) {
         const uint32_t pe = min(total, base + (uint32_t)ROUND);
         for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
@@ -1698,6 +1749,7 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
         const uint32_t total = kWalks * nrow * nb;
         for (uint32_t base = 0; base < total; base += ROUND) {
           const uint32_t pe = min(total, base + (uint32_t)ROUND);
+          const uint32_t mr = mainr;
           for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
             const uint32_t r2 = p / nb, jb = p - r2 * nb;
             const bool first = kWalks == 2 && r2 < nrow;
@@ -1714,6 +1766,8 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
             const uint32_t all = rowbits(bin0, x1 - x0 + 1), r0 = rank(bin0);
             uint32_t bits = all;
             if (trow) bits &= first ? touchbits(bx - 1 - x0) : ~touchbits(bx - 1 - x0);
+            // an item already in the dominant set: only the stragglers
+            if (mr != kNone && lfind(lp, na + jb) == mr) bits &= srowbits(bin0, x1 - x0 + 1);
             while (bits) {
               const int b = __builtin_ctz(bits);
               bits &= bits - 1;

@@ -1474,9 +1474,11 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
           if (cb != kNone) sB = G.start[cb], eB = G.start[cb + 1];
         }
         if (k >= 13 && eB - sB >= T) eB = sB;  // a dense backward cell links this pair itself
-        // runs of a cell sub-sorted into runs (kBigCell particles or more),
-        // particles of any other (small dense cells are left unsorted)
-        const bool dn = (eB - sB >= max(T, kBigCell));
+        // runs of a cell sub-sorted into runs, particles of any other: dense
+        // cells of kBigCell particles or more always are; the smaller ones
+        // only where k_link_sparse scans runs (sparse addressing or a
+        // variable l)
+        const bool dn = (eB - sB >= T) && (eB - sB >= kBigCell || G.sparse || V.vntab != 0);
         nbS[k] = sB;
         nbDense[k] = dn ? 1 : 0;
         items = dn ? rtoff[sB] : eB - sB;  // runs of a dense cell, else particles
@@ -3275,8 +3277,19 @@ static int enqueue_run(Run &r, hipStream_t st) {
   const uint32_t gbig = (uint32_t)std::min<uint64_t>(N / kBigCell + 1, pl.big_grid);
   bool any_sub = false;
   for (uint32_t v = 0; v < nvol; v++) any_sub = any_sub || pl.hv[v].d.m[0] > 0;
+  // constant l on a dense grid: k_link_sparse skips every dense cell, so only
+  // the cells k_link_dense links need runs; otherwise it scans dense forward
+  // cells run by run, and the smaller dense cells are sub-sorted too
+  bool fast = !pl.sparse;
+  for (uint32_t v = 0; v < nvol; v++) fast = fast && pl.hv[v].d.vntab == 0 && pl.hv[v].d.m[0] > 0;
   if (any_sub) {
     if ((rc = tb(4))) return rc;
+    if (!fast) {
+      k_subsort<256><<<gsml, 256, 0, st>>>(dense, small + kSmNDense, start, dV, dVB, (int)nvol,
+                                           pos4, reinterpret_cast<float4 *>(sk0), par, runend,
+                                           fine, rtoff, rtab);
+      FOF_HIP(hipGetLastError());
+    }
     k_subsort<256><<<gbig, 256, 0, st>>>(dbig, small + kSmNBig, start, dV, dVB, (int)nvol,
                                                 pos4, reinterpret_cast<float4 *>(sk0), par, runend,
                                                 fine, rtoff, rtab);
@@ -3285,8 +3298,6 @@ static int enqueue_run(Run &r, hipStream_t st) {
   }
   // 5-6: linking
   if ((rc = tb(5))) return rc;
-  bool fast = !pl.sparse;
-  for (uint32_t v = 0; v < nvol; v++) fast = fast && pl.hv[v].d.vntab == 0 && pl.hv[v].d.m[0] > 0;
   if (fast)
     k_link_sparse<true><<<gN, 256, 0, st>>>(NN, pos4, G, runend, fine, par, dV, dVB, (int)nvol, T, ctr);
   else

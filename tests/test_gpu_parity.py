@@ -429,3 +429,37 @@ def test_varlink_subcells_follow_occupied_cells(ctx):
     r2max, tab = lc.varlink_table()
     olab, _ = O.link_uf(Q["Pos"], odomain(v), O.varlink([0.0, 0.0, 0.0], r2max, tab))
     assert np.array_equal(lab, olab)
+
+
+@pytest.mark.parametrize("T,flags", [(16, 0), (24, 0), (16, L.OPT_SPARSE)])
+def test_small_dense_cell_sizes(ctx, T, flags):
+    # one cell per size around the boundaries of the all-pairs kernel for
+    # dense cells below 256 particles (64-particle tiles, 8-particle blocks)
+    # and of the sub-cell run kernel from 256: half of each cell's particles
+    # in tight blobs (cliques), half uniform at about one linking length
+    # apart, so cells hold several components that neighbours merge and the
+    # neighbour queues overflow; with sparse addressing the smaller dense
+    # cells are sub-sorted for the sparse kernel's run scan
+    rng = np.random.default_rng(31)
+    sizes = [16, 23, 24, 25, 63, 64, 65, 71, 127, 128, 129, 191, 192, 193, 254, 255, 256, 257]
+    nx, ny, nz, lc = 4, 4, 3, 1.0
+    cells = [(i, j, k) for k in range(nz) for j in range(ny) for i in range(nx)]
+    pos = []
+    for c, (i, j, k) in enumerate(cells):
+        n = sizes[c % len(sizes)]
+        lo = np.array([i, j, k], np.float64) * lc
+        nb = n // 2
+        centres = lo + 0.2 + rng.random((3, 3)) * 0.6
+        blob = centres[rng.integers(0, 3, nb)] + rng.normal(0, 0.03, (nb, 3))
+        uni = lo + rng.random((n - nb, 3)) * lc
+        pos.append(np.clip(np.concatenate([blob, uni]), lo, lo + lc * 0.999))
+    pos = np.concatenate(pos)
+    n = len(pos)
+    P = np.zeros(n, L.PART_DTYPE)
+    P["Pos"] = pos[rng.permutation(n)].astype(np.float32)
+    P["Vel"] = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    P["ID"] = np.arange(n)
+    box = [nx * lc, ny * lc, nz * lc]
+    dom = L.domain([0] * 3, box, [0] * 3, box, [lc] * 3, 0.2, 5)
+    lab, st = check_against_oracle(ctx, P, dom, T=T, flags=flags)
+    assert st.ndense > 0 and st.pair_tests_kernel[2] > 0 and st.pair_tests_kernel[1] > 0

@@ -90,11 +90,12 @@ def candidate_pairs(recs_u8, vols, T):
     (src_v2/FOF.c:94-161: every pair inside a cell plus every pair with the
     13 forward neighbours, on the reference grid of :45-65), per volume,
     charged to the kernel that tests them.  Dense cells (at least T
-    particles, sub-cells enabled) belong to k_link_dense from BIG_CELL
-    particles and to k_link_small below; for a constant
-    linking length a pair touching a dense cell is tested by that cell's
-    kernel (the lower cell's when both are dense), otherwise by the home
-    cell's.  Returns ({"big": .., "small": .., "sparse": ..}, dense_cells)."""
+    particles) belong to k_link_small below BIG_CELL particles and, with
+    sub-cells enabled, to k_link_dense from it; for a constant linking
+    length with sub-cells a pair touching a dense cell is tested by that
+    cell's kernel (the lower cell's when both are dense), otherwise by the
+    home cell's.  Returns ({"big": .., "small": .., "sparse": ..},
+    dense_cells)."""
     import torch
 
     xyz = recs_u8.view(torch.float32).view(-1, 8)[:, 2:5]
@@ -109,7 +110,8 @@ def candidate_pairs(recs_u8, vols, T):
         s = d.linklength * (1 - 1e-6) / np.sqrt(3.0)
         m = [max(1, int(np.ceil(d.lcell[k] / s))) if s > 0 else 99 for k in range(3)]
         sub_on = all(m[k] <= 12 and dims[k] * m[k] < 2 ** 21 for k in range(3))
-        by_dense = not bool(d.varlink)  # constant l: pairs with a dense cell go to its kernel
+        # constant l with sub-cells: pairs with a dense cell go to its kernel
+        by_dense = not bool(d.varlink) and sub_on
         idx = torch.empty(n, dtype=torch.int64, device=xyz.device)
         for c0 in range(0, n, 1 << 26):
             c1 = min(n, c0 + (1 << 26))
@@ -121,8 +123,8 @@ def candidate_pairs(recs_u8, vols, T):
         del idx
         # class of every cell: 0 sparse, 1 small dense, 2 big dense
         cls = torch.zeros_like(cnt, dtype=torch.int8)
+        cls[(cnt >= T) & (cnt < BIG_CELL)] = 1  # k_link_small, with or without sub-cells
         if sub_on:
-            cls[cnt >= T] = 1
             cls[(cnt >= T) & (cnt >= BIG_CELL)] = 2
         nd += int((cls > 0).sum())
         acc = torch.zeros(3, dtype=torch.float64, device=cnt.device)

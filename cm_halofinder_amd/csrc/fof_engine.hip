@@ -833,9 +833,9 @@ __global__ __launch_bounds__(kPcThreads) void k_pc_sort(uint64_t C, const uint32
     if (e < nc) {
       const uint32_t s = lc[P(e)], n = lc[P(e + 1)] - s;
       start[c0 + e] = q0 + s;
-      if (n >= T && vols[vol_of_cell(vols, nvol, (uint32_t)(c0 + e))].m[0] > 0) {
-        big = n >= kBigCell;
-        sml = !big;
+      if (n >= T) {  // dense: below kBigCell always, from it only with sub-cells
+        sml = n < kBigCell;
+        big = !sml && vols[vol_of_cell(vols, nvol, (uint32_t)(c0 + e))].m[0] > 0;
       }
     }
     append(0, sml, (uint32_t)(c0 + e));
@@ -956,9 +956,9 @@ __global__ __launch_bounds__(256) void k_cells_finish(
         i = (i + 1) & G.hmask;
       }
       const VolDev &V = vols[vol_of_sorted(vbeg, nvol, s, hint)];  // s ascends with c
-      if (V.m[0] > 0 && e - s >= T) {
-        big = e - s >= kBigCell;
-        sml = !big;
+      if (e - s >= T) {  // dense: below kBigCell always, from it only with sub-cells
+        sml = e - s < kBigCell;
+        big = !sml && V.m[0] > 0;
       }
     }
     const unsigned long long bs = __ballot(sml), bb = __ballot(big);
@@ -1046,6 +1046,7 @@ __global__ __launch_bounds__(NT) void k_subsort(const uint32_t *__restrict__ den
     const uint32_t c = dense[d];
     const uint32_t s = start[c], e = start[c + 1], n = e - s;
     const VolDev &V = vols[vol_of(vbeg, nvol, s)];
+    if (V.m[0] == 0) continue;  // no sub-cells in this volume (uniform per block)
     const int nbins = V.m[0] * V.m[1] * V.m[2];
     for (int b = threadIdx.x; b < kMaxCellBins; b += NT) bcur[b] = 0;
     __syncthreads();
@@ -1126,7 +1127,9 @@ __global__ __launch_bounds__(256, 8) void k_link_sparse(uint32_t N, const float4
     const CellPos cp = cell_pos(V, px.x, px.y, px.z);
     const uint32_t c = G.sparse ? G.cell_of[x] : cell_id(V, cp.ix, cp.iy, cp.iz);
     const uint32_t s = G.start[c], e = G.start[c + 1];
-    if (V.m[0] > 0 && e - s >= T) continue;  // dense: k_link_dense
+    // dense cells are linked by k_link_small (below kBigCell particles) and
+    // k_link_dense (from it, with sub-cells)
+    if (e - s >= T && (e - s < kBigCell || V.m[0] > 0)) continue;
     const Rule R = make_rule(V, cp.ix, cp.iy, cp.iz);
     uint32_t rx = x;  // x's root, refreshed by every join
     const int nx = V.nx, ny = V.ny, nz = V.nz;
@@ -2218,7 +2221,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_link_small(
       int dd[3];
       nb_offset((int)lane, dd);
       const int bx = cp.ix + dd[0], by = cp.iy + dd[1], bz = cp.iz + dd[2];
-      const bool want = lane < 13 || (!G.sparse && V.vntab == 0);
+      const bool want = lane < 13 || (!G.sparse && V.vntab == 0 && V.m[0] > 0);
       if (want && !(bx < 0 || bx >= V.nx || by < 0 || by >= V.ny || bz < 0 || bz >= V.nz)) {
         const uint32_t cb = cell_index(G, V, bx, by, bz);
         if (cb != kNone) nbS = G.start[cb], nbN = G.start[cb + 1] - nbS;
@@ -2761,9 +2764,30 @@ int prepare_volume(const fof_volume_v1 &in, double band, HostVol &hv, std::vecto
   V.band = (float)band;
   V.lo = (float)(V.linksq * (1.0 - band));
   V.hi = (float)(V.linksq * (1.0 + band));
-  // src_v2/FOF.c:63-65
+  // An internal grid finer than the reference's, for a constant l well
+  // inside the reference cells.  The reference predicate links a pair only
+  // if its true distance is below l(1 + 1e-7) (dx = fl32(xi - xj) errs by
+  // < 6e-8).  On an axis where lcell > l(1 + 1e-6), every such pair is at
+  // most one cell apart on the reference grid and on any grid whose cells
+  // are longer still, so both sweeps find the same linked pairs and the
+  // groups do not depend on the choice.  Such an axis is cut into
+  // r = ceil(lcell / 6.9 l) cells (so sub-cells, m <= 12, stay on: the
+  // all-pairs scan of coarse cells is what a coarse grid otherwise costs).
+  // A variable l keeps the reference grid: its cell's l^2 is part of the
+  // rule (src_v2/FOF.c:102-109).
+  if (!D.varlink && D.linklength > 0) {
+    for (int k = 0; k < 3; k++) {
+      const double L = D.lcell[k], l = D.linklength;
+      if (!(L > l * (1.0 + 1e-6))) continue;
+      double r = ceil(L / (6.9 * l));
+      const double ext = D.rmax_buff[k] - D.rmin_buff[k];
+      while (r > 1 && ceil(ext / (L / r)) + 1.0 > 1048576.0) r -= 1;  // at most 2^20 cells
+      if (r > 1) V.lcell[k] = L / r;
+    }
+  }
+  // src_v2/FOF.c:63-65 (on the grid in use)
   double e[3];
-  for (int k = 0; k < 3; k++) e[k] = ceil((D.rmax_buff[k] - D.rmin_buff[k]) / D.lcell[k]) + 1.0;
+  for (int k = 0; k < 3; k++) e[k] = ceil((D.rmax_buff[k] - D.rmin_buff[k]) / V.lcell[k]) + 1.0;
   if (!(e[0] < 2e9 && e[1] < 2e9 && e[2] < 2e9)) return FOF_E_RANGE;
   V.nx = (int)e[0], V.ny = (int)e[1], V.nz = (int)e[2];
   hv.ncell_d = e[0] * e[1] * e[2];
@@ -3319,12 +3343,13 @@ static int enqueue_run(Run &r, hipStream_t st) {
             ctr, small + kSmOverflow, ra);
     FOF_HIP(hipGetLastError());
     if ((rc = te(6))) return rc;
-    if ((rc = tb(15))) return rc;
-    k_link_small<<<(gsml + kSmallWaves - 1) / kSmallWaves, 64 * kSmallWaves, 0, st>>>(
-        dense, small + kSmNDense, pos4, G, T, par, dV, dVB, (int)nvol, ctr);
-    FOF_HIP(hipGetLastError());
-    if ((rc = te(15))) return rc;
   }
+  // dense cells below kBigCell particles, with or without sub-cells
+  if ((rc = tb(15))) return rc;
+  k_link_small<<<(gsml + kSmallWaves - 1) / kSmallWaves, 64 * kSmallWaves, 0, st>>>(
+      dense, small + kSmNDense, pos4, G, T, par, dV, dVB, (int)nvol, ctr);
+  FOF_HIP(hipGetLastError());
+  if ((rc = te(15))) return rc;
   // 7: union-find finish (minorig reuses runend, hsize reuses key, the
   // members' minimum ID reuses fine)
   uint32_t *minorig = runend, *hsize = key;

@@ -431,15 +431,20 @@ def test_varlink_subcells_follow_occupied_cells(ctx):
     assert np.array_equal(lab, olab)
 
 
-@pytest.mark.parametrize("T,flags", [(16, 0), (24, 0), (16, L.OPT_SPARSE)])
-def test_small_dense_cell_sizes(ctx, T, flags):
+@pytest.mark.parametrize("T,flags,l", [(16, 0, 0.2), (24, 0, 0.2), (16, L.OPT_SPARSE, 0.2),
+                                       (16, 0, 0.05), (16, L.OPT_SPARSE, 0.05)])
+def test_small_dense_cell_sizes(ctx, T, flags, l):
     # one cell per size around the boundaries of the all-pairs kernel for
     # dense cells below 256 particles (64-particle tiles, 8-particle blocks)
     # and of the sub-cell run kernel from 256: half of each cell's particles
     # in tight blobs (cliques), half uniform at about one linking length
     # apart, so cells hold several components that neighbours merge and the
     # neighbour queues overflow; with sparse addressing the smaller dense
-    # cells are sub-sorted for the sparse kernel's run scan
+    # cells are sub-sorted for the sparse kernel's run scan.  At l = 0.05,
+    # given as a (constant) variable-l table so that the reference grid is
+    # kept, the volume has no sub-cells (m > 12): the cells below 256
+    # particles still go to the all-pairs kernel, larger ones to the sparse
+    # kernel
     rng = np.random.default_rng(31)
     sizes = [16, 23, 24, 25, 63, 64, 65, 71, 127, 128, 129, 191, 192, 193, 254, 255, 256, 257]
     nx, ny, nz, lc = 4, 4, 3, 1.0
@@ -460,6 +465,29 @@ def test_small_dense_cell_sizes(ctx, T, flags):
     P["Vel"] = rng.normal(0, 1, (n, 3)).astype(np.float32)
     P["ID"] = np.arange(n)
     box = [nx * lc, ny * lc, nz * lc]
-    dom = L.domain([0] * 3, box, [0] * 3, box, [lc] * 3, 0.2, 5)
-    lab, st = check_against_oracle(ctx, P, dom, T=T, flags=flags)
-    assert st.ndense > 0 and st.pair_tests_kernel[2] > 0 and st.pair_tests_kernel[1] > 0
+    vl = ovl = None
+    if l < 0.1:
+        r2max = float(np.sum(np.square(box)))
+        tab = np.full(8, l * l)
+        vl = L.VarLinkTable([0.0, 0.0, 0.0], r2max, tab)
+        ovl = O.varlink([0.0, 0.0, 0.0], r2max, tab)
+    dom = L.domain([0] * 3, box, [0] * 3, box, [lc] * 3, l, 5, vl)
+    lab, st = check_against_oracle(ctx, P, dom, T=T, vl=vl, ovl=ovl, flags=flags)
+    assert st.ndense > 0 and st.pair_tests_kernel[2] > 0
+    assert (st.pair_tests_kernel[1] > 0) == (l > 0.1)  # runs only with sub-cells
+
+
+@pytest.mark.parametrize("coarse", [2.5, 6.0])
+def test_coarse_grid_refined(ctx, coarse):
+    # reference cells several linking lengths across (m > 12 on the
+    # reference grid): with a constant l the engine links on a finer grid of
+    # its own; the oracle sweeps the reference grid, and the groups agree
+    bx = W.plain(24, seed=8)
+    _, P, vols = W.host_domain(bx)
+    d = vols[0].domain
+    lc = [d.lcell[k] * coarse for k in range(3)]
+    dom = L.domain(list(d.rmin), list(d.rmax), list(d.rmin_buff), list(d.rmax_buff), lc,
+                   d.linklength, d.nphalomin)
+    lab, st = check_against_oracle(ctx, P, dom)
+    ref_cells = np.prod([np.ceil((d.rmax_buff[k] - d.rmin_buff[k]) / lc[k]) + 1 for k in range(3)])
+    assert st.ncells > ref_cells and st.ndense > 0

@@ -298,7 +298,10 @@ typedef struct fof_plan_v1 {
   uint32_t sparse;          /* 1: occupied-cell addressing                   */
   uint64_t n;               /* particles                                     */
   uint64_t cells;           /* dense cell table entries (0 when sparse)      */
-  double grid_cells;        /* cells of the reference grids, all volumes     */
+  double grid_cells;        /* cells of the grids linked on, all volumes: the
+                               reference grid, or a finer one on the axes
+                               where a constant l is below lcell (1 - 1e-6)
+                               (the same linked pairs; see fof_link_gpu)    */
   uint64_t workspace_bytes; /* device workspace of the call                  */
   uint32_t dense_threshold; /* in effect                                     */
   uint32_t reserved;

@@ -179,3 +179,18 @@ def test_plan_batch_workspace_grows_with_volumes():
     p4 = L.plan(fake_particles(4000), vols)
     p1 = L.plan(fake_particles(1000), vols[:1])
     assert p4.cells == 4 * p1.cells and p4.workspace_bytes > p1.workspace_bytes
+
+
+def test_plan_internal_grid():
+    # a constant l far inside the reference cells: each such axis is cut into
+    # ceil(lcell / 6.9 l) cells (the same linked pairs, sub-cells stay on);
+    # l at or above lcell (1 - 1e-6), or a variable l, keeps the reference grid
+    n = 1 << 16
+    assert L.plan(fake_particles(n), [volume(n, 100.0, 2.0)]).grid_cells == 101.0 ** 3  # 2 a side
+    assert L.plan(fake_particles(n), [volume(n, 100.0, 3.0)]).grid_cells == 101.0 ** 3  # 3 a side
+    assert L.plan(fake_particles(n), [volume(n, 100.0, 1.0)]).grid_cells == 101.0 ** 3  # as given
+    assert L.plan(fake_particles(n), [volume(n, 100.0, 0.2)]).grid_cells == 501.0 ** 3  # l = lcell
+    v = volume(n, 100.0, 3.0)
+    vl = L.VarLinkTable([0.0, 0.0, 0.0], 3e4, np.full(4, 0.04))
+    v.domain = L.domain([0] * 3, [100.0] * 3, [0] * 3, [100.0] * 3, [3.0] * 3, 0.2, 20, vl)
+    assert L.plan(fake_particles(n), [v]).grid_cells == 35.0 ** 3  # ceil(100/3) + 1

@@ -19,6 +19,7 @@ ABI_VERSION = 1
 MEM_HOST, MEM_DEVICE = 0, 1
 OPT_TIMING, OPT_OUT_DEVICE, OPT_NO_RING, OPT_NO_HALOS = 0x1, 0x2, 0x4, 0x8
 OPT_GRAPH, OPT_SPARSE, OPT_AUTOTUNE = 0x10, 0x20, 0x40
+OPT_WRITE_PAIRS, OPT_WRITE_DIRECT = 0x80, 0x100
 NOALLOC_BYTES = 256 << 20  # FOF_NOALLOC_BYTES
 
 E_ARG, E_DOMAIN, E_RANGE, E_NOMEM, E_HIP, E_NODEV = -1, -2, -3, -4, -5, -6

@@ -113,6 +113,14 @@ typedef struct fof_domain_v1 {
                                    input (once per shape; results are cached
                                    per context and in the file named by the
                                    environment variable FOF_TUNE_CACHE)       */
+#define FOF_OPT_WRITE_PAIRS 0x80u   /* write labels and the ring through a
+                                       partition of the entries by index (the
+                                       default for a volume of more than 2^25
+                                       particles, whose random writes would
+                                       spread beyond the last-level cache)    */
+#define FOF_OPT_WRITE_DIRECT 0x100u /* write them straight by index (the
+                                       default up to 2^25 particles a volume);
+                                       with both flags, WRITE_PAIRS holds      */
 
 typedef struct fof_opts_v1 {
   uint32_t version;

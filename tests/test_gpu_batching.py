@@ -167,3 +167,15 @@ def test_pipelines_against_oracle(ctx):
         oh = O.retrieve(O.ring_from_labels(olab), P["Pos"][a:b], P["Vel"][a:b], od)
         h = got["hal"][int(got["hoff"][v]):int(got["hoff"][v + 1])]
         assert np.array_equal(h["head"], oh["head"]) and np.array_equal(h["sum_pos"], oh["sum_pos"])
+
+
+@pytest.mark.parametrize("flags", [L.OPT_WRITE_PAIRS, L.OPT_WRITE_DIRECT,
+                                   L.OPT_WRITE_PAIRS | L.OPT_WRITE_DIRECT])
+def test_label_and_ring_write_modes(ctx, flags):
+    # labels and ring written straight by index (the default at this size)
+    # or through the index partition (the default for volumes above 2^25
+    # particles): the same outputs
+    P, vols = batch_case(seed=58)
+    ref, _ = host_call(ctx, P, vols, L.opts())
+    got, _ = host_call(ctx, P, vols, L.opts(flags))
+    assert same(ref, got)

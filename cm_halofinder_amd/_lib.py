@@ -115,7 +115,7 @@ assert HALO_DTYPE.itemsize == C.sizeof(Halo) == 64
 
 NPHASE = 17
 PHASES = ("keys", "scan", "scatter", "uf_init", "subsort", "link_sparse", "link_dense",
-          "flatten", "label", "compact", "ring_sort", "ring", "halo_list", "halo_sums", "output",
+          "flatten", "label", "groups", "group_sort", "ring", "halo_list", "halo_sums", "output",
           "link_small", "total")
 
 

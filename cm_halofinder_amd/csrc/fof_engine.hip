@@ -26,8 +26,11 @@
 //                    pair tested from LDS broadcasts, components by BFS over
 //                    a bit matrix in registers
 //   7 union-find finish: flatten, group minimum index, sizes
-//   8 labels (written by index, straight or through scatter_u32),
-//     compaction of kept groups, stable radix sort by label
+//   8 groups of two or more ranked by label through a label bitmap, each
+//     given a segment of the member list; labels (written by index,
+//     straight or through scatter_u32) while every member goes to its
+//     group's segment; each segment sorted by index (a wave in registers,
+//     or a workgroup)
 //   9 ring in ascending index order (written the same way), halo list,
 //     FP64 halo sums
 //
@@ -425,12 +428,12 @@ __device__ __forceinline__ void add_counters(unsigned long long *ctr, int ktests
 // the counters follow at word kSmCtr.
 constexpr int kSmErr = 0;       // bit 0: particle outside [rmin_buff, rmax_buff]
 constexpr int kSmNDense = 1;    // dense cells below kBigCell particles
-constexpr int kSmM = 2;         // members of kept groups (label-sorted list)
+constexpr int kSmM = 2;         // members of groups of two or more (member list)
 constexpr int kSmH = 3;         // halos
 constexpr int kSmOverflow = 4;  // run-table overflow
 constexpr int kSmU = 5;         // occupied cells (sparse addressing)
 constexpr int kSmNBig = 7;      // dense cells from kBigCell particles
-constexpr int kSmPairs = 8;     // (index, label) pairs of group members
+constexpr int kSmR = 8;         // groups of two or more (their roots)
 constexpr int kSmNtB = 10;      // tiles of the fine partition pass
 constexpr int kSmCtr = 64;
 
@@ -523,17 +526,9 @@ __global__ __launch_bounds__(256) void k_pa_count(const uint8_t *__restrict__ ba
 #pragma unroll
     for (int k = 0; k < kCnt; k++) {
       const uint32_t p = q0 + threadIdx.x + k * 256;
-      const bool valid = p < p1;
-      const uint32_t b = c[k] >> (kFineBits + 8);
-      if (valid) key[p] = c[k];
-      // lanes of one bucket add once (a wave's particles mostly share a
-      // coarse bucket, and same-address LDS atomics serialise)
-      unsigned long long todo = __ballot(valid);
-      while (todo) {
-        const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)b, (int)__builtin_ctzll(todo));
-        const unsigned long long same = __ballot(valid && b == b0);
-        if ((threadIdx.x & 63) == (uint32_t)__builtin_ctzll(same)) atomicAdd(&h[b0], (uint32_t)__popcll(same));
-        todo &= ~same;
+      if (p < p1) {
+        key[p] = c[k];
+        atomicAdd(&h[c[k] >> (kFineBits + 8)], 1u);
       }
     }
   }
@@ -2101,19 +2096,42 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
   }
   const uint32_t total = __shfl(off, 25, 64);
   off -= nbN;
-  // the cell's box: a neighbour particle farther than l from it links to
-  // none of the cell's particles (prune2 carries a 1e-4 margin on l^2)
+  // the bounding box of the cell's particles: a neighbour particle farther
+  // than l from it links to none of them (prune2 carries a 1e-4 margin on
+  // l^2; the gap is evaluated in FP64 from the float coordinates)
   double lo[3], hi[3];
-  const int ic[3] = {cp.ix, cp.iy, cp.iz};
+  {
+    float mn[3] = {kFar, kFar, kFar}, mx[3] = {-kFar, -kFar, -kFar};
 #pragma unroll
-  for (int a = 0; a < 3; a++) {
-    lo[a] = V.rmin_buff[a] + ic[a] * V.lcell[a];
-    hi[a] = lo[a] + V.lcell[a];
+    for (int r = 0; r < NW; r++) {
+      const uint32_t i = 64 * r + lane;
+      if (i < nA) {
+        const float4 q = S.apos[i];
+        mn[0] = fminf(mn[0], q.x), mn[1] = fminf(mn[1], q.y), mn[2] = fminf(mn[2], q.z);
+        mx[0] = fmaxf(mx[0], q.x), mx[1] = fmaxf(mx[1], q.y), mx[2] = fmaxf(mx[2], q.z);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        mn[a] = fminf(mn[a], __shfl_xor(mn[a], o, 64));
+        mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o, 64));
+      }
+#pragma unroll
+    for (int a = 0; a < 3; a++) lo[a] = mn[a], hi[a] = mx[a];
   }
-  // index g of the space -> particle
-  auto locate = [&](uint32_t g) -> uint32_t {
-    uint32_t base = 0;
-    for (int k = 0; k < 26; k++) {
+  // index g of the space -> particle, for the 64 consecutive indices g0 +
+  // lane: only the ranges from the one holding g0 to the one holding the
+  // chunk's last index are scanned
+  auto locate = [&](uint32_t g0) -> uint32_t {
+    const uint32_t g = g0 + lane, gl = min(g0 + 63u, total - 1u);
+    const unsigned long long la = __ballot(lane < 26 && off <= g0);
+    const unsigned long long lb = __ballot(lane < 26 && off <= gl);
+    const int ka = 63 - __clzll(la), kb = 63 - __clzll(lb);
+    uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)nbS, ka) -
+                    (uint32_t)__builtin_amdgcn_readlane((int)off, ka);
+    for (int k = ka + 1; k <= kb; k++) {
       const uint32_t ok = (uint32_t)__builtin_amdgcn_readlane((int)off, k);
       const uint32_t sk = (uint32_t)__builtin_amdgcn_readlane((int)nbS, k);
       if (g >= ok) base = sk - ok;
@@ -2211,7 +2229,8 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
   for (uint32_t g0 = 0; g0 < total; g0 += 128) {
     const uint32_t ga = g0 + lane, gb = g0 + 64 + lane;
     const bool va = ga < total, vb = gb < total;
-    const uint32_t ja = va ? locate(ga) : 0u, jb = vb ? locate(gb) : 0u;
+    const uint32_t ja = locate(g0);
+    const uint32_t jb = g0 + 64 < total ? locate(g0 + 64) : 0u;
     const float4 pa = va ? ld4(pos4 + ja) : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 pb = vb ? ld4(pos4 + jb) : make_float4(0.f, 0.f, 0.f, 0.f);
     push(pa, ja, va);
@@ -2358,16 +2377,18 @@ __global__ __launch_bounds__(256) void k_flatten(uint32_t N, uint32_t *__restric
 }
 
 // Members of groups of two or more: label[orig] = group minimum index |
-// kFlag.  The head's entry of hsize receives the group size.  Two ways to
-// write the index-ordered label array from the cell-ordered particles:
+// kFlag, and each member's (label, orig) at the next free slot of its group
+// in the member list (the group's cursor, indexed by its label, starts
+// at the group's first slot; lanes of one group take consecutive slots with
+// one atomic).  The head's entry of hsize receives the group size.  Two ways
+// to write the index-ordered label array from the cell-ordered particles:
 //  DIRECT  random 4-byte writes straight into label -- when every volume's
 //          label range is small enough to stay in the last-level cache
 //          while its cells are processed (the 4^3-batched 1024^3 box: 11 ms
-//          less than the pairs);
-//  pairs   (orig, label) pairs gathered per tile of kLabTile particles in
-//          LDS and appended with one atomic per tile, for scatter_u32 to
-//          partition by index before writing (one 1.09e9-particle volume:
-//          4 ms less than direct writes spread over 4.4 GB).
+//          less than the partitioned writes);
+//  else    scatter_u32 from the member list afterwards, partitioned by index
+//          (one 1.09e9-particle volume: 4 ms less than direct writes spread
+//          over 4.4 GB).
 constexpr uint32_t kLabTile = 4096;
 // Largest volume (particles) written directly: 128 MB of labels.
 constexpr uint64_t kDirectMax = 1ull << 25;
@@ -2378,28 +2399,24 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
                                                const uint32_t *__restrict__ minorig,
                                                const uint32_t *__restrict__ gsize,
                                                uint32_t *__restrict__ out,
-                                               uint32_t *__restrict__ pval,
-                                               uint32_t *__restrict__ npairs,
                                                uint32_t *__restrict__ hsize,
+                                               uint32_t *__restrict__ cursor,
+                                               uint32_t *__restrict__ mkey,
+                                               uint32_t *__restrict__ mval,
                                                const uint8_t *__restrict__ base, uint32_t stride,
                                                uint32_t offid,
                                                const unsigned long long *__restrict__ idroot,
                                                unsigned long long *__restrict__ idhead) {
-  __shared__ uint32_t lk[DIRECT ? 1 : kLabTile], lv[DIRECT ? 1 : kLabTile];
-  __shared__ uint32_t ln, lbase;
   const uint32_t lane = threadIdx.x & 63;
-  if (!DIRECT) {
-    if (threadIdx.x == 0) ln = 0;
-    __syncthreads();
-  }
+  const unsigned long long lt = (1ull << lane) - 1;
   const uint64_t x0 = (uint64_t)blockIdx.x * kLabTile;
   for (uint32_t q = 0; q < kLabTile; q += 256) {
     const uint64_t xx = x0 + q + threadIdx.x;
     bool mem = false;
-    uint32_t o = 0, lab = 0;
+    uint32_t o = 0, lab = 0, r = 0;
     if (xx < N) {
       const uint32_t x = (uint32_t)xx;
-      const uint32_t r = par[x];
+      r = par[x];
       const uint32_t g = gsize[r];
       if (!(r == x && g == 0)) {  // singletons: k_label_init wrote them
         mem = true;
@@ -2416,25 +2433,29 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
         }
       }
     }
-    if (DIRECT) continue;
-    const unsigned long long b = __ballot(mem);
-    if (!b) continue;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&ln, (uint32_t)__popcll(b));
-    base = __shfl(base, 0, 64);
-    if (mem) {
-      const uint32_t j = base + (uint32_t)__popcll(b & ((1ull << lane) - 1));
-      lk[j] = o;
-      lv[j] = lab | kFlag;
+    // member-list slots: the wave's first few groups one atomic each (a
+    // wave's particles are mostly in one or two groups), any other lane its own
+    unsigned long long todo = __ballot(mem);
+    for (int it = 0; it < 4 && todo; it++) {
+      const int first = __builtin_ctzll(todo);
+      const uint32_t l0 = (uint32_t)__builtin_amdgcn_readlane((int)lab, first);
+      const bool mine = mem && lab == l0;
+      const unsigned long long same = __ballot(mine);
+      uint32_t b = 0;
+      if ((int)lane == first) b = atomicAdd(cursor + l0, (uint32_t)__popcll(same));
+      b = (uint32_t)__builtin_amdgcn_readlane((int)b, first);
+      if (mine) {
+        const uint32_t slot = b + (uint32_t)__popcll(same & lt);
+        mkey[slot] = lab;
+        mval[slot] = o;
+      }
+      todo &= ~same;
     }
-  }
-  if (DIRECT) return;
-  __syncthreads();
-  if (threadIdx.x == 0) lbase = ln ? atomicAdd(npairs, ln) : 0u;
-  __syncthreads();
-  for (uint32_t j = threadIdx.x; j < ln; j += 256) {
-    out[lbase + j] = lk[j];
-    pval[lbase + j] = lv[j];
+    if ((todo >> lane) & 1ull) {
+      const uint32_t slot = atomicAdd(cursor + lab, 1u);
+      mkey[slot] = lab;
+      mval[slot] = o;
+    }
   }
 }
 
@@ -2505,6 +2526,265 @@ __global__ __launch_bounds__(256) void k_out_init(uint32_t N, const uint32_t *__
 // volume-local successors), and halo starts.  The entries go straight into
 // next_out, or, with pidx, leave as (index, successor) pairs in list order
 // for scatter_u32 (the choice of k_label).
+// ---------------------------------------------------------------------------
+// The member list: every group of two or more as one segment, segments in
+// ascending label (head index) order, members inside a segment in ascending
+// input index -- the order of the reference's rings (src_v2/FOF.c:359-373
+// joins rings so that each runs up the input order from its head) and of its
+// halo loops (src_v2/FOF.c:797-847).  Built without sorting the members by
+// label: the groups' roots are sorted by label (one entry per group), a scan
+// of their sizes gives each segment's first slot, k_label puts every member
+// in its segment, and each segment is then sorted by index on its own.
+// ---------------------------------------------------------------------------
+// The group list in label order without a sort: each group of two or more
+// sets the bit of its label (the group's lowest input index) in a bitmap over
+// the input indices and puts its size at hsize[label]; a scan of the words'
+// bit counts ranks the labels.
+__global__ __launch_bounds__(256) void k_group_bits(uint32_t N, const uint32_t *__restrict__ par,
+                                                    const uint32_t *__restrict__ gsize,
+                                                    const uint32_t *__restrict__ minorig,
+                                                    const float4 *__restrict__ pos4,
+                                                    uint32_t *__restrict__ bits,
+                                                    uint32_t *__restrict__ hsize) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
+    if (par[j] != j) continue;
+    const uint32_t g = gsize[j];
+    if (g == 0) continue;  // a singleton
+    const uint32_t lab = min(minorig[j], __float_as_uint(pos4[j].w));
+    atomicOr(bits + (lab >> 5), 1u << (lab & 31));
+    hsize[lab] = g + 1u;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_word_counts(uint64_t nw, const uint32_t *__restrict__ bits,
+                                                     uint32_t *__restrict__ cnt) {
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
+       w += (uint64_t)gridDim.x * blockDim.x)
+    cnt[w] = (uint32_t)__popc(bits[w]);
+}
+
+// Labels in ascending order and their groups' sizes (scanned in place into
+// first slots); the group count to the device counts.
+__global__ __launch_bounds__(256) void k_group_list(uint64_t nw, const uint32_t *__restrict__ bits,
+                                                    const uint32_t *__restrict__ wpre,
+                                                    const uint32_t *__restrict__ hsize,
+                                                    uint32_t *__restrict__ glab,
+                                                    uint32_t *__restrict__ offs,
+                                                    uint32_t *__restrict__ dR) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *dR = wpre[nw];
+  for (uint64_t w = t; w < nw; w += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t m = bits[w], at = wpre[w];
+    while (m) {
+      const uint32_t lab = (uint32_t)(w * 32) + (uint32_t)__builtin_ctz(m);
+      m &= m - 1;
+      glab[at] = lab;
+      offs[at] = hsize[lab];
+      at++;
+    }
+  }
+}
+
+// Each group's cursor (indexed by its label) at its first slot; the member
+// count to the device counts.
+__global__ __launch_bounds__(256) void k_group_cursors(const uint32_t *__restrict__ dR,
+                                                       const uint32_t *__restrict__ glab,
+                                                       const uint32_t *__restrict__ offs,
+                                                       uint32_t *__restrict__ cursor,
+                                                       uint32_t *__restrict__ dM) {
+  const uint32_t R = *dR;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *dM = offs[R];
+  for (uint32_t i = t; i < R; i += gridDim.x * blockDim.x) cursor[glab[i]] = offs[i];
+}
+
+// Sorting each segment of the member list by index.  Groups of at most
+// kSegWave members are sorted by one wave each in registers (a bitonic
+// network; 64 E elements for E per lane, lane-major, so that the stages
+// below distance E stay inside a lane and the others are lane shuffles);
+// larger ones are listed for k_seg_sort_large: a workgroup each, a bitonic
+// sort in LDS up to kSegLds members, beyond that an LSD radix sort (four
+// 8-bit digits, stable scatter per 256-element tile) through scratch.
+constexpr uint32_t kSegWave = 1024;
+constexpr uint32_t kSegLds = 4096;
+constexpr int kSmLarge = 9;  // groups listed for k_seg_sort_large
+
+// n <= 64: one element a lane, the network only as wide as n needs
+__device__ __forceinline__ void seg_sort_lane(uint32_t *__restrict__ p, uint32_t n, uint32_t lane) {
+  uint32_t v = lane < n ? p[lane] : 0xFFFFFFFFu;
+  uint32_t P = 2;
+  while (P < n) P <<= 1;
+  for (uint32_t k = 2; k <= P; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      const uint32_t u = (uint32_t)__shfl_xor((int)v, (int)j, 64);
+      const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+      v = (lower == up) ? min(v, u) : max(v, u);
+    }
+  if (lane < n) p[lane] = v;
+}
+
+// 32 E < n <= 64 E: element a = E lane + e in register e of the lane
+template <int E>
+__device__ __forceinline__ void seg_sort_regs(uint32_t *__restrict__ p, uint32_t n, uint32_t lane) {
+  constexpr uint32_t P = 64u * E;
+  uint32_t v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint32_t a = lane * E + e;
+    v[e] = a < n ? p[a] : 0xFFFFFFFFu;
+  }
+#pragma unroll
+  for (uint32_t k = 2; k <= P; k <<= 1)
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j < (uint32_t)E) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          if (e & j) continue;
+          const bool up = ((lane * E + e) & k) == 0;
+          const uint32_t x = v[e], y = v[e | j];
+          v[e] = up ? min(x, y) : max(x, y);
+          v[e | j] = up ? max(x, y) : min(x, y);
+        }
+      } else {
+        const uint32_t lj = j / E;
+        const bool lower = (lane & lj) == 0;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const uint32_t u = (uint32_t)__shfl_xor((int)v[e], (int)lj, 64);
+          const bool up = ((lane * E + e) & k) == 0;
+          v[e] = (lower == up) ? min(v[e], u) : max(v[e], u);
+        }
+      }
+    }
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint32_t a = lane * E + e;
+    if (a < n) p[a] = v[e];
+  }
+}
+
+// One wave per group; larger groups are appended to the list for
+// k_seg_sort_large.
+__global__ __launch_bounds__(256) void k_seg_sort(const uint32_t *__restrict__ dR,
+                                                  const uint32_t *__restrict__ offs,
+                                                  uint32_t *__restrict__ mval,
+                                                  uint32_t *__restrict__ large,
+                                                  uint32_t *__restrict__ dL) {
+  const uint32_t R = *dR;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t w0 = blockIdx.x * 4 + (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (uint32_t g = w0; g < R; g += gridDim.x * 4) {
+    const uint32_t s = offs[g], n = offs[g + 1] - s;
+    uint32_t *p = mval + s;
+    if (n <= 64) seg_sort_lane(p, n, lane);
+    else if (n <= 128) seg_sort_regs<2>(p, n, lane);
+    else if (n <= 256) seg_sort_regs<4>(p, n, lane);
+    else if (n <= 512) seg_sort_regs<8>(p, n, lane);
+    else if (n <= kSegWave) seg_sort_regs<16>(p, n, lane);
+    else if (lane == 0) large[atomicAdd(dL, 1u)] = g;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_seg_sort_large(const uint32_t *__restrict__ dL,
+                                                        const uint32_t *__restrict__ large,
+                                                        const uint32_t *__restrict__ offs,
+                                                        uint32_t *__restrict__ mval,
+                                                        uint32_t *__restrict__ scratch) {
+  __shared__ uint32_t sk[kSegLds];
+  __shared__ uint32_t hist[256], wc[4][256], wsum[4];
+  const uint32_t nl = *dL;
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const unsigned long long lt = (1ull << lane) - 1;
+  for (uint32_t q = blockIdx.x; q < nl; q += gridDim.x) {
+    const uint32_t g = large[q];
+    const uint32_t s = offs[g], n = offs[g + 1] - s;
+    if (n <= kSegLds) {
+      uint32_t P = 128;
+      while (P < n) P <<= 1;
+      __syncthreads();
+      for (uint32_t i = t; i < P; i += 256) sk[i] = i < n ? mval[s + i] : 0xFFFFFFFFu;
+      __syncthreads();
+      for (uint32_t k = 2; k <= P; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+          for (uint32_t i = t; i < P / 2; i += 256) {
+            const uint32_t a = ((i & ~(j - 1)) << 1) | (i & (j - 1)), b = a + j;
+            const uint32_t x = sk[a], y = sk[b];
+            if ((x > y) == ((a & k) == 0)) sk[a] = y, sk[b] = x;
+          }
+          __syncthreads();
+        }
+      for (uint32_t i = t; i < n; i += 256) mval[s + i] = sk[i];
+      continue;
+    }
+    uint32_t *src = mval + s, *dst = scratch + s;
+    for (int shift = 0; shift < 32; shift += 8) {
+      __syncthreads();
+      hist[t] = 0;
+      __syncthreads();
+      for (uint32_t i = t; i < n; i += 256) atomicAdd(&hist[(src[i] >> shift) & 255u], 1u);
+      __syncthreads();
+      {  // exclusive scan of the 256 digit counts: the digits' cursors
+        const uint32_t c = hist[t];
+        uint32_t inc = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t u = __shfl_up(inc, o, 64);
+          if (lane >= (uint32_t)o) inc += u;
+        }
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        uint32_t pre = inc - c;
+        for (uint32_t q2 = 0; q2 < w; q2++) pre += wsum[q2];
+        hist[t] = pre;
+      }
+      for (uint32_t b0 = 0; b0 < n; b0 += 256) {
+        const uint32_t i = b0 + t;
+        const bool valid = i < n;
+        const uint32_t key = valid ? src[i] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+          const bool on = (d >> bit) & 1u;
+          const unsigned long long bb = __ballot(on);
+          peers &= on ? bb : ~bb;
+        }
+        __syncthreads();  // the previous tile's reads of wc and hist are done
+        wc[w][lane] = 0, wc[w][lane + 64] = 0, wc[w][lane + 128] = 0, wc[w][lane + 192] = 0;
+        __syncthreads();
+        if (valid && (peers & lt) == 0) wc[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+          uint32_t at = hist[d] + (uint32_t)__popcll(peers & lt);
+          for (uint32_t q2 = 0; q2 < w; q2++) at += wc[q2][d];
+          dst[at] = key;
+        }
+        __syncthreads();
+        hist[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+      }
+      uint32_t *tmp = src;
+      src = dst, dst = tmp;
+    }
+  }
+}
+
+// Halo flags of the groups in label order: the first slot | kFlag for a
+// group passing its volume's size cut (src_v2/FOF.c:290-298), else 0.
+__global__ __launch_bounds__(256) void k_halo_flags(const uint32_t *__restrict__ dR,
+                                                    const uint32_t *__restrict__ rkey,
+                                                    const uint32_t *__restrict__ offs,
+                                                    const VolDev *__restrict__ vols,
+                                                    const uint32_t *__restrict__ vbeg, int nvol,
+                                                    uint32_t *__restrict__ hsrc) {
+  const uint32_t R = *dR;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x) {
+    const uint32_t n = offs[i + 1] - offs[i];
+    const double nph = vols[vol_of(vbeg, nvol, rkey[i])].nphalomin;
+    hsrc[i] = !((double)n < nph) ? (offs[i] | kFlag) : 0u;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
                                               const uint32_t *__restrict__ skey,
                                               const uint32_t *__restrict__ sval,
@@ -2523,8 +2803,10 @@ __global__ __launch_bounds__(256) void k_ring(const uint32_t *__restrict__ dM,
     const uint32_t nx = (k + 1 < M && skey[k + 1] == lab) ? sval[k + 1] : lab;
     if (pidx) pidx[k] = i, pval[k] = nx - b + 1u;
     else if (next_out) next_out[i] = nx - b + 1u;
-    const bool gstart = (k == 0) || (skey[k - 1] != lab);
-    hflag[k] = (gstart && !((double)hsize[lab] < nphalomin)) ? kFlag : 0u;
+    if (hflag) {
+      const bool gstart = (k == 0) || (skey[k - 1] != lab);
+      hflag[k] = (gstart && !((double)hsize[lab] < nphalomin)) ? kFlag : 0u;
+    }
   };
   // the list is sorted by label, so by volume; a member shares its head's
   // volume, and a tile whose first and last labels share one has no other
@@ -3455,61 +3737,92 @@ static int enqueue_run(Run &r, hipStream_t st) {
   k_flatten<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, ctr, dbase, stride, offid, idroot);
   FOF_HIP(hipGetLastError());
   if ((rc = te(7))) return rc;
-  if ((rc = tb(8))) return rc;
+  // 8: the group list -- groups of two or more in label order, each
+  // group's first slot in the member list -- then the labels, which also
+  // fill the list.  In the sort buffers' upper half: the label bitmap, its
+  // word prefix counts and the first slots (sk1), the labels in order (sv1);
+  // a group has two members or more, so there are at most N / 2 groups.
+  const uint64_t rcap = std::max<uint64_t>(N / 2, 1);
+  const uint64_t nw = (N + 31) / 32;
+  uint32_t *gbits = sk1, *wpre = sk1 + nw, *offs = sk1 + 2 * nw + 1;
+  uint32_t *glist = sv1;               // labels of the groups, ascending
+  uint32_t *aux = sv1 + q4 / 2;        // large groups, then halo flags
+  uint32_t *cursor = rtab;             // per label, from its group's first slot
+  uint32_t *p32 = reinterpret_cast<uint32_t *>(pos4);  // free after k_label
+  if ((rc = tb(9))) return rc;
   k_label_init<<<tiles_for(N), 256, 0, st>>>(NN, dV, dVB, (int)nvol, label, hsize);
   FOF_HIP(hipGetLastError());
+  if ((rc = fill_u32(gbits, nw, 0u, st))) return rc;
+  k_group_bits<<<gN, 256, 0, st>>>(NN, par, gsize, minorig, pos4, gbits, hsize);
+  FOF_HIP(hipGetLastError());
+  k_word_counts<<<grid_for(nw), 256, 0, st>>>(nw, gbits, wpre);
+  FOF_HIP(hipGetLastError());
+  if ((rc = scan_u32(wpre, wpre, nw, B.tmp, st))) return rc;
+  k_group_list<<<grid_for(nw), 256, 0, st>>>(nw, gbits, wpre, hsize, glist, offs, small + kSmR);
+  FOF_HIP(hipGetLastError());
+  if ((rc = scan_u32_count(offs, offs, rcap, small + kSmR, B.tmp, st))) return rc;
+  k_group_cursors<<<grid_for(rcap), 256, 0, st>>>(small + kSmR, glist, offs, cursor, small + kSmM);
+  FOF_HIP(hipGetLastError());
+  if ((rc = te(9))) return rc;
+  if ((rc = tb(8))) return rc;
   const bool direct = pl.direct_writes;
   const uint32_t glab = (uint32_t)((N + kLabTile - 1) / kLabTile);
-  if (direct) {
-    k_label<true><<<glab, 256, 0, st>>>(NN, par, pos4, minorig, gsize, label, nullptr, nullptr,
-                                        hsize, dbase, stride, offid, idroot, B.idhead);
-    FOF_HIP(hipGetLastError());
-  } else {
-    k_label<false><<<glab, 256, 0, st>>>(NN, par, pos4, minorig, gsize, sk0, sv0,
-                                         small + kSmPairs, hsize, dbase, stride, offid, idroot,
-                                         B.idhead);
-    FOF_HIP(hipGetLastError());
-    if ((rc = scatter_u32(sk0, sv0, sk1, sv1, N, small + kSmPairs, N, label, B.tmp, st))) return rc;
-  }
+  if (direct)
+    k_label<true><<<glab, 256, 0, st>>>(NN, par, pos4, minorig, gsize, label, hsize, cursor, sk0,
+                                        sv0, dbase, stride, offid, idroot, B.idhead);
+  else
+    k_label<false><<<glab, 256, 0, st>>>(NN, par, pos4, minorig, gsize, label, hsize, cursor, sk0,
+                                         sv0, dbase, stride, offid, idroot, B.idhead);
+  FOF_HIP(hipGetLastError());
+  if (!direct && (rc = scatter_u32(sv0, sk0, p32, p32 + N, N, small + kSmM, N, label, kFlag,
+                                   B.tmp, st)))
+    return rc;
   k_out_init<<<tiles_for(N), 256, 0, st>>>(NN, label, dVB, (int)nvol, r.d_label, r.d_next, hsize,
                                            dbase, stride, offid, B.idhead, r.d_minid);
   FOF_HIP(hipGetLastError());
   if ((rc = te(8))) return rc;
-  // 8: kept groups sorted by label (stable: ascending index inside a group)
-  if ((rc = tb(9))) return rc;
-  if ((rc = compact_flagged(label, N, nullptr, kFlag, sk0, sv0, small + kSmM, B.tmp, st))) return rc;
-  if ((rc = te(9))) return rc;
+  // members in index order inside each group (large groups through the
+  // free cell-sorted positions)
   if ((rc = tb(10))) return rc;
-  if ((rc = radix_sort_u32(sk0, sv0, sk1, sv1, N, small + kSmM, bits_for(N), B.tmp, st))) return rc;
+  k_seg_sort<<<(uint32_t)std::min<uint64_t>((rcap + 3) / 4, 16384), 256, 0, st>>>(
+      small + kSmR, offs, sv0, aux, small + kSmLarge);
+  FOF_HIP(hipGetLastError());
+  k_seg_sort_large<<<(uint32_t)std::min<uint64_t>(N / kSegWave + 1, 8192), 256, 0, st>>>(
+      small + kSmLarge, aux, offs, sv0, p32);
+  FOF_HIP(hipGetLastError());
   if ((rc = te(10))) return rc;
-  // 9: ring + halos
+  // 9: the rings
   if ((rc = tb(11))) return rc;
   {
-    // ring pairs (the partitioned way) in the now free cell-sorted positions
-    uint32_t *p32 = reinterpret_cast<uint32_t *>(pos4);
+    // ring pairs (the partitioned way) in the free cell-sorted positions
     uint32_t *pidx = (r.d_next && !direct) ? p32 : nullptr;
     k_ring<<<tiles_for(N), 256, 0, st>>>(small + kSmM, sk0, sv0, hsize, dV, dVB, (int)nvol,
-                                        r.d_next, pidx, p32 + N, sk1);
+                                        r.d_next, pidx, p32 + N, nullptr);
     FOF_HIP(hipGetLastError());
     if (pidx && (rc = scatter_u32(p32, p32 + N, p32 + 2 * N, p32 + 3 * N, N, small + kSmM, N,
-                                  r.d_next, B.tmp, st)))
+                                  r.d_next, 0u, B.tmp, st)))
       return rc;
   }
   if ((rc = te(11))) return rc;
   if (pl.want_halos) {
-    // halo starts: positions k of the heads of groups passing the size cut
+    // halo starts: first slots of the groups passing the size cut, in label
+    // order (the flags go where the roots were)
     if ((rc = tb(12))) return rc;
-    if ((rc = compact_flagged(sk1, N, small + kSmM, kFlag, nullptr, sv1, small + kSmH, B.tmp, st)))
+    k_halo_flags<<<grid_for(rcap), 256, 0, st>>>(small + kSmR, glist, offs, dV, dVB, (int)nvol,
+                                                  aux);
+    FOF_HIP(hipGetLastError());
+    if ((rc = compact_flagged(aux, rcap, small + kSmR, kFlag, p32, p32 + N, small + kSmH, B.tmp,
+                              st)))
       return rc;
     if ((rc = te(12))) return rc;
     if ((rc = tb(13))) return rc;
     const uint32_t gh = (uint32_t)std::min<uint64_t>((pl.Hbound * 64 + 255) / 256, 8192);
     if (r.d_mom)
-      k_halo_sums<true><<<gh, 256, 0, st>>>(small + kSmH, sv1, sk0, sv0, hsize, dbase, stride,
+      k_halo_sums<true><<<gh, 256, 0, st>>>(small + kSmH, p32, sk0, sv0, hsize, dbase, stride,
                                             offpos, offvel, dV, dVB, (int)nvol, r.d_halo, r.d_mom,
                                             r.hcap, B.hhead, ctr);
     else
-      k_halo_sums<false><<<gh, 256, 0, st>>>(small + kSmH, sv1, sk0, sv0, hsize, dbase, stride,
+      k_halo_sums<false><<<gh, 256, 0, st>>>(small + kSmH, p32, sk0, sv0, hsize, dbase, stride,
                                              offpos, offvel, dV, dVB, (int)nvol, r.d_halo, nullptr,
                                              r.hcap, B.hhead, ctr);
     FOF_HIP(hipGetLastError());

@@ -57,6 +57,9 @@ void release(Scratch &s);
 size_t scan_tmp_bytes(uint64_t n);
 int scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, void *tmp,
              hipStream_t st);
+// The same over min(*dn, n) values (the count left on the device).
+int scan_u32_count(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *dn, void *tmp,
+                   hipStream_t st);
 
 // Passes over a list whose length an earlier kernel left in device memory
 // take the host-known capacity n and dn (the device count, n when dn is
@@ -69,13 +72,14 @@ size_t sort_tmp_bytes(uint64_t n);
 int radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1,
                    uint64_t n, const uint32_t *dn, int bits, void *tmp, hipStream_t st);
 
-// out[idx[i]] = val[i] for i in [0,n), every idx[i] < nout: the pairs are
-// first partitioned (one radix pass) by the top 8 bits of idx, so that the
-// writes of a stretch of the list fall in one 1/256 of out and merge in the
-// caches (random 4-byte writes cost one memory request each).  (idx2, val2)
-// receive the partitioned pairs; tmp: sort_tmp_bytes(n).
-int scatter_u32(uint32_t *idx, uint32_t *val, uint32_t *idx2, uint32_t *val2, uint64_t n,
-                const uint32_t *dn, uint64_t nout, uint32_t *out, void *tmp, hipStream_t st);
+// out[idx[i]] = val[i] | orv for i in [0,n), every idx[i] < nout: the pairs
+// are first partitioned (one radix pass) by the top 8 bits of idx, so that
+// the writes of a stretch of the list fall in one 1/256 of out and merge in
+// the caches (random 4-byte writes cost one memory request each).  (idx2,
+// val2) receive the partitioned pairs; tmp: sort_tmp_bytes(n).
+int scatter_u32(const uint32_t *idx, const uint32_t *val, uint32_t *idx2, uint32_t *val2,
+                uint64_t n, const uint32_t *dn, uint64_t nout, uint32_t *out, uint32_t orv,
+                void *tmp, hipStream_t st);
 
 // Stable compaction: for every i in [0,n) with (src[i] & flag) != 0, in
 // ascending i, append key = src[i] & ~flag and value = i.  The count goes to

@@ -373,6 +373,11 @@ int scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, void *tmp, hipStream
   return scan_u32_dev(in, out, n, nullptr, tmp, st);
 }
 
+int scan_u32_count(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *dn, void *tmp,
+                   hipStream_t st) {
+  return scan_u32_dev(in, out, n, dn, tmp, st);
+}
+
 size_t compact_tmp_bytes(uint64_t n) { return scan_levels_bytes(n) + 256; }
 
 int compact_flagged(const uint32_t *src, uint64_t n, const uint32_t *dn, uint32_t flag,
@@ -443,18 +448,20 @@ int radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint6
 // them over many partitions).
 __global__ __launch_bounds__(kBlock) void k_put(const uint32_t *__restrict__ idx,
                                                 const uint32_t *__restrict__ val, uint64_t ncap,
-                                                const uint32_t *dn, uint32_t *__restrict__ out) {
+                                                const uint32_t *dn, uint32_t orv,
+                                                uint32_t *__restrict__ out) {
   const uint64_t n = dev_count(dn, ncap);
   const uint64_t base = (uint64_t)blockIdx.x * kTile;
 #pragma unroll
   for (int r = 0; r < kItems; r++) {
     const uint64_t i = base + (uint64_t)r * kBlock + threadIdx.x;
-    if (i < n) out[idx[i]] = val[i];
+    if (i < n) out[idx[i]] = val[i] | orv;
   }
 }
 
-int scatter_u32(uint32_t *idx, uint32_t *val, uint32_t *idx2, uint32_t *val2, uint64_t n,
-                const uint32_t *dn, uint64_t nout, uint32_t *out, void *tmp, hipStream_t st) {
+int scatter_u32(const uint32_t *idx, const uint32_t *val, uint32_t *idx2, uint32_t *val2,
+                uint64_t n, const uint32_t *dn, uint64_t nout, uint32_t *out, uint32_t orv,
+                void *tmp, hipStream_t st) {
   if (n == 0) return 0;
   int bits = 0;
   while (bits < 32 && (nout - 1) >> bits) bits++;
@@ -469,7 +476,7 @@ int scatter_u32(uint32_t *idx, uint32_t *val, uint32_t *idx2, uint32_t *val2, ui
   if (rc) return rc;
   k_rs_scatter<<<(uint32_t)nb, kBlock, 0, st>>>(idx, val, idx2, val2, n, dn, shift, hist);
   FOF_HIP(hipGetLastError());
-  k_put<<<(uint32_t)nb, kBlock, 0, st>>>(idx2, val2, n, dn, out);
+  k_put<<<(uint32_t)nb, kBlock, 0, st>>>(idx2, val2, n, dn, orv, out);
   FOF_HIP(hipGetLastError());
   return 0;
 }

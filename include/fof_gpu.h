@@ -166,8 +166,8 @@ typedef struct fof_halo_v1 {
  *     of 256 particles or more
  *   6 pair tests, dense cells of    7 union-find flatten + group minima
  *     256 particles or more
- *   8 labels                        9 compaction of kept groups
- *  10 radix sort by label          11 ring
+ *   8 labels, member list filled    9 label defaults + group list
+ *  10 members sorted in each group 11 ring
  *  12 halo list                    13 halo sums
  *  14 output copies                15 pair tests, dense cells below 256
  *  16 total, first span to last                                             */
@@ -182,7 +182,7 @@ typedef struct fof_stats_v1 {
   uint64_t pair_tests;    /* particle-pair distance tests                    */
   uint64_t rechecks;      /* tests re-decided in FP64                        */
   uint64_t ngroups;       /* groups of any size                              */
-  uint64_t nmembers;      /* particles in groups kept for the ring sort      */
+  uint64_t nmembers;      /* particles in groups of two or more (the ring)  */
   uint64_t nhalos;        /* groups passing the size cut                     */
   uint64_t nhalos_owned;  /* of those, owned (Checkhalo)                     */
   uint32_t sparse;        /* 1 if occupied-cell addressing was used          */

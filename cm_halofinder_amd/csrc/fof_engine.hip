@@ -1191,8 +1191,14 @@ __global__ __launch_bounds__(256, 8) void k_link_sparse(uint32_t N, const float4
         ok[r] = ry >= 0 && ry < ny && rz < nz;
         const uint32_t c0 = ok[r] ? cell_id(V, cp.ix - xl, ry, rz) : 0u;
         const int ncell = xl + 1 + xr;
-#pragma unroll
-        for (int k = 0; k < 4; k++) b[r][k] = ok[r] ? G.start[c0 + min(k, ncell)] : 0u;
+        // the four cell starts from c0 in one load (the table has slack
+        // behind its last entry)
+        uint32_t q[4];
+        __builtin_memcpy(q, G.start + c0, sizeof(q));
+        b[r][0] = ok[r] ? q[0] : 0u;
+        b[r][1] = ok[r] ? q[1] : 0u;
+        b[r][2] = ok[r] ? (ncell >= 2 ? q[2] : q[1]) : 0u;
+        b[r][3] = ok[r] ? (ncell >= 3 ? q[3] : b[r][2]) : 0u;
       }
 #pragma unroll
       for (int r = 0; r < 5; r++) {
@@ -2664,8 +2670,10 @@ __device__ __forceinline__ void seg_sort_regs(uint32_t *__restrict__ p, uint32_t
   }
 }
 
-// One wave per group; larger groups are appended to the list for
-// k_seg_sort_large.
+// A wave takes 64 consecutive groups at a time: groups of at most four
+// members are sorted by their lane alone (most groups are pairs), the others
+// one at a time by the whole wave; groups above kSegWave members are
+// appended to the list for k_seg_sort_large.
 __global__ __launch_bounds__(256) void k_seg_sort(const uint32_t *__restrict__ dR,
                                                   const uint32_t *__restrict__ offs,
                                                   uint32_t *__restrict__ mval,
@@ -2674,15 +2682,37 @@ __global__ __launch_bounds__(256) void k_seg_sort(const uint32_t *__restrict__ d
   const uint32_t R = *dR;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t w0 = blockIdx.x * 4 + (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  for (uint32_t g = w0; g < R; g += gridDim.x * 4) {
-    const uint32_t s = offs[g], n = offs[g + 1] - s;
-    uint32_t *p = mval + s;
-    if (n <= 64) seg_sort_lane(p, n, lane);
-    else if (n <= 128) seg_sort_regs<2>(p, n, lane);
-    else if (n <= 256) seg_sort_regs<4>(p, n, lane);
-    else if (n <= 512) seg_sort_regs<8>(p, n, lane);
-    else if (n <= kSegWave) seg_sort_regs<16>(p, n, lane);
-    else if (lane == 0) large[atomicAdd(dL, 1u)] = g;
+  for (uint32_t c0 = w0 * 64; c0 < R; c0 += gridDim.x * 4 * 64) {
+    const uint32_t g = c0 + lane;
+    uint32_t s = 0, n = 0;
+    if (g < R) s = offs[g], n = offs[g + 1] - s;
+    if (n <= 4) {  // a comparator network on the lane's own group
+      uint32_t v[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) v[e] = (uint32_t)e < n ? mval[s + e] : 0xFFFFFFFFu;
+      auto cas = [&](int a, int b) {
+        const uint32_t x = v[a], y = v[b];
+        v[a] = min(x, y), v[b] = max(x, y);
+      };
+      cas(0, 1), cas(2, 3), cas(0, 2), cas(1, 3), cas(1, 2);
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if ((uint32_t)e < n) mval[s + e] = v[e];
+    }
+    unsigned long long big = __ballot(n > 4);
+    while (big) {
+      const int l = __builtin_ctzll(big);
+      big &= big - 1;
+      const uint32_t sb = (uint32_t)__builtin_amdgcn_readlane((int)s, l);
+      const uint32_t nb = (uint32_t)__builtin_amdgcn_readlane((int)n, l);
+      uint32_t *p = mval + sb;
+      if (nb <= 64) seg_sort_lane(p, nb, lane);
+      else if (nb <= 128) seg_sort_regs<2>(p, nb, lane);
+      else if (nb <= 256) seg_sort_regs<4>(p, nb, lane);
+      else if (nb <= 512) seg_sort_regs<8>(p, nb, lane);
+      else if (nb <= kSegWave) seg_sort_regs<16>(p, nb, lane);
+      else if (lane == 0) large[atomicAdd(dL, 1u)] = c0 + (uint32_t)l;
+    }
   }
 }
 
@@ -3784,7 +3814,7 @@ static int enqueue_run(Run &r, hipStream_t st) {
   // members in index order inside each group (large groups through the
   // free cell-sorted positions)
   if ((rc = tb(10))) return rc;
-  k_seg_sort<<<(uint32_t)std::min<uint64_t>((rcap + 3) / 4, 16384), 256, 0, st>>>(
+  k_seg_sort<<<(uint32_t)std::min<uint64_t>((rcap + 255) / 256, 8192), 256, 0, st>>>(
       small + kSmR, offs, sv0, aux, small + kSmLarge);
   FOF_HIP(hipGetLastError());
   k_seg_sort_large<<<(uint32_t)std::min<uint64_t>(N / kSegWave + 1, 8192), 256, 0, st>>>(

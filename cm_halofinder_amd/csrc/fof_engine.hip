@@ -2412,7 +2412,9 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
                                                const uint8_t *__restrict__ base, uint32_t stride,
                                                uint32_t offid,
                                                const unsigned long long *__restrict__ idroot,
-                                               unsigned long long *__restrict__ idhead) {
+                                               unsigned long long *__restrict__ idhead, int single,
+                                               const VolDev *__restrict__ vols,
+                                               const uint32_t *__restrict__ vbeg, int nvol) {
   const uint32_t lane = threadIdx.x & 63;
   const unsigned long long lt = (1ull << lane) - 1;
   const uint64_t x0 = (uint64_t)blockIdx.x * kLabTile;
@@ -2424,7 +2426,9 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
       const uint32_t x = (uint32_t)xx;
       r = par[x];
       const uint32_t g = gsize[r];
-      if (!(r == x && g == 0)) {  // singletons: k_label_init wrote them
+      // singletons: k_label_init wrote them, unless their volume keeps
+      // groups of one (those join the member list)
+      if (!(r == x && g == 0) || (single && !(1.0 < vols[vol_of(vbeg, nvol, x)].nphalomin))) {
         mem = true;
         o = __float_as_uint(pos4[x].w);
         const uint32_t orr = (r == x) ? o : __float_as_uint(pos4[r].w);
@@ -2543,22 +2547,25 @@ __global__ __launch_bounds__(256) void k_out_init(uint32_t N, const uint32_t *__
 // in its segment, and each segment is then sorted by index on its own.
 // ---------------------------------------------------------------------------
 // The group list in label order without a sort: each group of two or more
-// sets the bit of its label (the group's lowest input index) in a bitmap over
-// the input indices and puts its size at hsize[label]; a scan of the words'
-// bit counts ranks the labels.
+// (and, where a volume keeps groups of one -- Nphalomin <= 1 -- each such
+// particle) sets the bit of its label (the group's lowest input index) in a
+// bitmap over the input indices and puts its size at lsize[label]; a scan of
+// the words' bit counts ranks the labels.
 __global__ __launch_bounds__(256) void k_group_bits(uint32_t N, const uint32_t *__restrict__ par,
                                                     const uint32_t *__restrict__ gsize,
                                                     const uint32_t *__restrict__ minorig,
                                                     const float4 *__restrict__ pos4,
                                                     uint32_t *__restrict__ bits,
-                                                    uint32_t *__restrict__ hsize) {
+                                                    uint32_t *__restrict__ lsize, int single,
+                                                    const VolDev *__restrict__ vols,
+                                                    const uint32_t *__restrict__ vbeg, int nvol) {
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
     if (par[j] != j) continue;
     const uint32_t g = gsize[j];
-    if (g == 0) continue;  // a singleton
-    const uint32_t lab = min(minorig[j], __float_as_uint(pos4[j].w));
+    if (g == 0 && (!single || 1.0 < vols[vol_of(vbeg, nvol, j)].nphalomin)) continue;
+    const uint32_t lab = min(minorig[j], __float_as_uint(pos4[j].w));  // minorig: ~0 alone
     atomicOr(bits + (lab >> 5), 1u << (lab & 31));
-    hsize[lab] = g + 1u;
+    lsize[lab] = g + 1u;
   }
 }
 
@@ -2573,7 +2580,7 @@ __global__ __launch_bounds__(256) void k_word_counts(uint64_t nw, const uint32_t
 // first slots); the group count to the device counts.
 __global__ __launch_bounds__(256) void k_group_list(uint64_t nw, const uint32_t *__restrict__ bits,
                                                     const uint32_t *__restrict__ wpre,
-                                                    const uint32_t *__restrict__ hsize,
+                                                    const uint32_t *__restrict__ lsize,
                                                     uint32_t *__restrict__ glab,
                                                     uint32_t *__restrict__ offs,
                                                     uint32_t *__restrict__ dR) {
@@ -2585,7 +2592,7 @@ __global__ __launch_bounds__(256) void k_group_list(uint64_t nw, const uint32_t 
       const uint32_t lab = (uint32_t)(w * 32) + (uint32_t)__builtin_ctz(m);
       m &= m - 1;
       glab[at] = lab;
-      offs[at] = hsize[lab];
+      offs[at] = lsize[lab];
       at++;
     }
   }
@@ -3239,6 +3246,7 @@ struct Plan {
   uint32_t F = 0, Bc = 0, ntA = 0, ntBmax = 0;
   uint64_t nA = 0, nB = 0;
   uint64_t ndense_cap = 0, Hbound = 0, Hstage = 0;
+  bool single_halos = false;  // some volume keeps groups of one (Nphalomin <= 1)
   size_t tmpb = 0;
   // what the call produces
   uint32_t flags = 0;
@@ -3391,6 +3399,7 @@ static int make_plan(const fof_particles_v1 *P, const fof_volume_v1 *vols, uint3
   // kept groups have at least max(1, ceil(nphalomin)) members and are disjoint
   const double kmin = std::max(1.0, std::ceil(std::min(nphmin, 4e9)));
   pl.Hbound = std::min<uint64_t>(N, (uint64_t)((double)N / kmin) + 1);
+  pl.single_halos = !(1.0 < nphmin);
   pl.Hstage = std::min<uint64_t>(pl.Hbound, out->halo_cap);
   pl.tmpb = std::max(sort_tmp_bytes(N), compact_tmp_bytes(N));
   if (!pl.sparse) {
@@ -3767,31 +3776,36 @@ static int enqueue_run(Run &r, hipStream_t st) {
   k_flatten<<<gN, 256, 0, st>>>(NN, par, pos4, minorig, gsize, ctr, dbase, stride, offid, idroot);
   FOF_HIP(hipGetLastError());
   if ((rc = te(7))) return rc;
-  // 8: the group list -- groups of two or more in label order, each
-  // group's first slot in the member list -- then the labels, which also
-  // fill the list.  In the sort buffers' upper half: the label bitmap, its
-  // word prefix counts and the first slots (sk1), the labels in order (sv1);
-  // a group has two members or more, so there are at most N / 2 groups.
-  const uint64_t rcap = std::max<uint64_t>(N / 2, 1);
+  // 8: the group list -- groups of two or more (and kept groups of one) in
+  // label order, each group's first slot in the member list -- then the
+  // labels, which also fill the list.  The label bitmap and its word prefix
+  // counts live where the cursors will (rtab), the sizes by label in the
+  // label array (before its defaults), the first slots in sk1 and the labels
+  // in order in sv1 (behind 64 words: R + 1 first slots may reach N + 1);
+  // large groups, halo flags and the halo list go to the cell-sorted
+  // positions once k_label has read them.
+  const int single = pl.single_halos ? 1 : 0;
+  const uint64_t rcap = std::max<uint64_t>(single ? N : N / 2, 1);
   const uint64_t nw = (N + 31) / 32;
-  uint32_t *gbits = sk1, *wpre = sk1 + nw, *offs = sk1 + 2 * nw + 1;
-  uint32_t *glist = sv1;               // labels of the groups, ascending
-  uint32_t *aux = sv1 + q4 / 2;        // large groups, then halo flags
+  uint32_t *gbits = rtab, *wpre = rtab + nw, *lsize = label;
+  uint32_t *offs = sk1, *glist = sv1 + 64;
   uint32_t *cursor = rtab;             // per label, from its group's first slot
-  uint32_t *p32 = reinterpret_cast<uint32_t *>(pos4);  // free after k_label
+  uint32_t *p32 = reinterpret_cast<uint32_t *>(pos4);
+  uint32_t *large = p32 + 3 * N, *hflags = p32 + 2 * N;
   if ((rc = tb(9))) return rc;
-  k_label_init<<<tiles_for(N), 256, 0, st>>>(NN, dV, dVB, (int)nvol, label, hsize);
-  FOF_HIP(hipGetLastError());
   if ((rc = fill_u32(gbits, nw, 0u, st))) return rc;
-  k_group_bits<<<gN, 256, 0, st>>>(NN, par, gsize, minorig, pos4, gbits, hsize);
+  k_group_bits<<<gN, 256, 0, st>>>(NN, par, gsize, minorig, pos4, gbits, lsize, single, dV, dVB,
+                                   (int)nvol);
   FOF_HIP(hipGetLastError());
   k_word_counts<<<grid_for(nw), 256, 0, st>>>(nw, gbits, wpre);
   FOF_HIP(hipGetLastError());
   if ((rc = scan_u32(wpre, wpre, nw, B.tmp, st))) return rc;
-  k_group_list<<<grid_for(nw), 256, 0, st>>>(nw, gbits, wpre, hsize, glist, offs, small + kSmR);
+  k_group_list<<<grid_for(nw), 256, 0, st>>>(nw, gbits, wpre, lsize, glist, offs, small + kSmR);
   FOF_HIP(hipGetLastError());
   if ((rc = scan_u32_count(offs, offs, rcap, small + kSmR, B.tmp, st))) return rc;
   k_group_cursors<<<grid_for(rcap), 256, 0, st>>>(small + kSmR, glist, offs, cursor, small + kSmM);
+  FOF_HIP(hipGetLastError());
+  k_label_init<<<tiles_for(N), 256, 0, st>>>(NN, dV, dVB, (int)nvol, label, hsize);
   FOF_HIP(hipGetLastError());
   if ((rc = te(9))) return rc;
   if ((rc = tb(8))) return rc;
@@ -3799,10 +3813,12 @@ static int enqueue_run(Run &r, hipStream_t st) {
   const uint32_t glab = (uint32_t)((N + kLabTile - 1) / kLabTile);
   if (direct)
     k_label<true><<<glab, 256, 0, st>>>(NN, par, pos4, minorig, gsize, label, hsize, cursor, sk0,
-                                        sv0, dbase, stride, offid, idroot, B.idhead);
+                                        sv0, dbase, stride, offid, idroot, B.idhead, single, dV,
+                                        dVB, (int)nvol);
   else
     k_label<false><<<glab, 256, 0, st>>>(NN, par, pos4, minorig, gsize, label, hsize, cursor, sk0,
-                                         sv0, dbase, stride, offid, idroot, B.idhead);
+                                         sv0, dbase, stride, offid, idroot, B.idhead, single, dV,
+                                         dVB, (int)nvol);
   FOF_HIP(hipGetLastError());
   if (!direct && (rc = scatter_u32(sv0, sk0, p32, p32 + N, N, small + kSmM, N, label, kFlag,
                                    B.tmp, st)))
@@ -3815,10 +3831,10 @@ static int enqueue_run(Run &r, hipStream_t st) {
   // free cell-sorted positions)
   if ((rc = tb(10))) return rc;
   k_seg_sort<<<(uint32_t)std::min<uint64_t>((rcap + 255) / 256, 8192), 256, 0, st>>>(
-      small + kSmR, offs, sv0, aux, small + kSmLarge);
+      small + kSmR, offs, sv0, large, small + kSmLarge);
   FOF_HIP(hipGetLastError());
   k_seg_sort_large<<<(uint32_t)std::min<uint64_t>(N / kSegWave + 1, 8192), 256, 0, st>>>(
-      small + kSmLarge, aux, offs, sv0, p32);
+      small + kSmLarge, large, offs, sv0, p32);
   FOF_HIP(hipGetLastError());
   if ((rc = te(10))) return rc;
   // 9: the rings
@@ -3836,12 +3852,12 @@ static int enqueue_run(Run &r, hipStream_t st) {
   if ((rc = te(11))) return rc;
   if (pl.want_halos) {
     // halo starts: first slots of the groups passing the size cut, in label
-    // order (the flags go where the roots were)
+    // order
     if ((rc = tb(12))) return rc;
     k_halo_flags<<<grid_for(rcap), 256, 0, st>>>(small + kSmR, glist, offs, dV, dVB, (int)nvol,
-                                                  aux);
+                                                  hflags);
     FOF_HIP(hipGetLastError());
-    if ((rc = compact_flagged(aux, rcap, small + kSmR, kFlag, p32, p32 + N, small + kSmH, B.tmp,
+    if ((rc = compact_flagged(hflags, rcap, small + kSmR, kFlag, p32, p32 + N, small + kSmH, B.tmp,
                               st)))
       return rc;
     if ((rc = te(12))) return rc;

@@ -491,3 +491,38 @@ def test_coarse_grid_refined(ctx, coarse):
     lab, st = check_against_oracle(ctx, P, dom)
     ref_cells = np.prod([np.ceil((d.rmax_buff[k] - d.rmin_buff[k]) / lc[k]) + 1 for k in range(3)])
     assert st.ncells > ref_cells and st.ndense > 0
+
+
+@pytest.mark.parametrize("nph", [0.5, 1.0, 2.0])
+def test_size_cut_of_one(ctx, nph):
+    """Nphalomin <= 1 keeps groups of one: each is a halo and its own ring."""
+    bx = W.plain(20, seed=11, nphalomin=nph)
+    _, P, vols = W.host_domain(bx)
+    check_against_oracle(ctx, P, vols[0].domain)
+
+
+def test_size_cut_of_one_batched(ctx):
+    """Volumes keeping groups of one beside volumes that do not, in one call."""
+    bx = W.plain(24, seed=53)
+    recs = L.gen_host(bx.gen())
+    P, _, vols = L.ghosts_host(bx.ghost(2), recs)
+    for k, v in enumerate(vols):
+        v.domain.nphalomin = 1.0 if k % 2 == 0 else 20.0
+    n = len(P)
+    nxt, lab = np.empty(n, np.uint32), np.empty(n, np.uint32)
+    hal = np.empty(n, L.HALO_DTYPE)
+    hoff = np.zeros(len(vols) + 1, np.uint64)
+    L.check(ctx.link_batch(L.particles(P), vols, L.opts(), nxt, lab, hal, n, hoff))
+    for k, v in enumerate(vols):
+        a, b = v.offset, v.offset + v.n
+        Pv = P[a:b]
+        od = odomain(v.domain)
+        olab, _ = O.link_uf(Pv["Pos"], od, None)
+        assert np.array_equal(lab[a:b], olab), k
+        oring = O.ring_from_labels(olab)
+        assert np.array_equal(nxt[a:b].astype(np.int64) - 1, oring.astype(np.int64)), k
+        oh = O.retrieve(oring, Pv["Pos"], Pv["Vel"], od)
+        h = hal[int(hoff[k]):int(hoff[k + 1])]
+        assert len(h) == len(oh), k
+        for f in ("head", "n", "owned", "edge_warn", "sum_pos", "sum_vel"):
+            assert np.array_equal(h[f], oh[f]), (k, f)

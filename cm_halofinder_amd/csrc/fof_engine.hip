@@ -2391,7 +2391,9 @@ __global__ __launch_bounds__(256) void k_flatten(uint32_t N, uint32_t *__restric
 //  DIRECT  random 4-byte writes straight into label -- when every volume's
 //          label range is small enough to stay in the last-level cache
 //          while its cells are processed (the 4^3-batched 1024^3 box: 11 ms
-//          less than the partitioned writes);
+//          less than the partitioned writes); with label_out given (no
+//          minimum-ID output wanted) straight into the caller's output, as
+//          volume-local labels over the defaults k_label_init wrote;
 //  else    scatter_u32 from the member list afterwards, partitioned by index
 //          (one 1.09e9-particle volume: 4 ms less than direct writes spread
 //          over 4.4 GB).
@@ -2414,10 +2416,18 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
                                                const unsigned long long *__restrict__ idroot,
                                                unsigned long long *__restrict__ idhead, int single,
                                                const VolDev *__restrict__ vols,
-                                               const uint32_t *__restrict__ vbeg, int nvol) {
+                                               const uint32_t *__restrict__ vbeg, int nvol,
+                                               uint32_t *__restrict__ label_out) {
   const uint32_t lane = threadIdx.x & 63;
   const unsigned long long lt = (1ull << lane) - 1;
   const uint64_t x0 = (uint64_t)blockIdx.x * kLabTile;
+  // the tile's volume base when the tile lies in one volume (volumes are
+  // contiguous in the cell order too), else looked up per particle
+  uint32_t btile = kNone;
+  if (DIRECT && label_out && x0 < N) {
+    const int va = vol_of(vbeg, nvol, (uint32_t)x0);
+    if (min((uint64_t)N, x0 + kLabTile) <= vbeg[va + 1]) btile = vbeg[va];
+  }
   for (uint32_t q = 0; q < kLabTile; q += 256) {
     const uint64_t xx = x0 + q + threadIdx.x;
     bool mem = false;
@@ -2433,7 +2443,10 @@ __global__ __launch_bounds__(256) void k_label(uint32_t N, const uint32_t *__res
         o = __float_as_uint(pos4[x].w);
         const uint32_t orr = (r == x) ? o : __float_as_uint(pos4[r].w);
         lab = min(minorig[r], orr);
-        if (DIRECT) out[o] = lab | kFlag;
+        if (DIRECT && label_out)  // straight to the output, volume-local
+          label_out[o] = lab - (btile != kNone ? btile : vbeg[vol_of(vbeg, nvol, x)]);
+        else if (DIRECT)
+          out[o] = lab | kFlag;
         if (o == lab) {
           hsize[lab] = g + 1u;
           if (idhead) {  // the root's own ID joins the members' minimum
@@ -2481,22 +2494,31 @@ static inline uint32_t tiles_for(uint64_t n) { return (uint32_t)std::max<uint64_
 __global__ __launch_bounds__(256) void k_label_init(uint32_t N, const VolDev *__restrict__ vols,
                                                     const uint32_t *__restrict__ vbeg, int nvol,
                                                     uint32_t *__restrict__ label,
-                                                    uint32_t *__restrict__ hsize) {
+                                                    uint32_t *__restrict__ hsize,
+                                                    uint32_t *__restrict__ label_out,
+                                                    uint32_t *__restrict__ next_out) {
   const uint64_t i0 = (uint64_t)blockIdx.x * kTileN, i1 = min((uint64_t)N, i0 + kTileN);
   if (i0 >= i1) return;
-  auto body = [&](uint32_t i, double nphalomin) {
-    label[i] = i | (!(1.0 < nphalomin) ? kFlag : 0u);
-    hsize[i] = 1u;
+  // internal defaults (label, hsize) or, when k_label writes label_out
+  // itself, the outputs' defaults: own label and own ring entry, 1-based
+  auto body = [&](uint32_t i, double nphalomin, uint32_t b) {
+    if (label) label[i] = i | (!(1.0 < nphalomin) ? kFlag : 0u);
+    if (hsize) hsize[i] = 1u;
+    if (label_out) label_out[i] = i - b;
+    if (next_out) next_out[i] = i - b + 1u;
   };
   const int va = vol_of(vbeg, nvol, (uint32_t)i0);
   if (i1 <= vbeg[va + 1]) {  // the tile lies in one volume
     const double nph = vols[va].nphalomin;
-    for (uint32_t i = (uint32_t)i0 + threadIdx.x; i < i1; i += blockDim.x) body(i, nph);
+    const uint32_t b = vbeg[va];
+    for (uint32_t i = (uint32_t)i0 + threadIdx.x; i < i1; i += blockDim.x) body(i, nph, b);
     return;
   }
   int hint = va;
-  for (uint32_t i = (uint32_t)i0 + threadIdx.x; i < i1; i += blockDim.x)
-    body(i, vols[vol_of_run(vbeg, nvol, i, N, hint)].nphalomin);
+  for (uint32_t i = (uint32_t)i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const int v = vol_of_run(vbeg, nvol, i, N, hint);
+    body(i, vols[v].nphalomin, vbeg[v]);
+  }
 }
 
 // Outputs for every particle: label_out (volume-local), for particles not in
@@ -3805,27 +3827,34 @@ static int enqueue_run(Run &r, hipStream_t st) {
   if ((rc = scan_u32_count(offs, offs, rcap, small + kSmR, B.tmp, st))) return rc;
   k_group_cursors<<<grid_for(rcap), 256, 0, st>>>(small + kSmR, glist, offs, cursor, small + kSmM);
   FOF_HIP(hipGetLastError());
-  k_label_init<<<tiles_for(N), 256, 0, st>>>(NN, dV, dVB, (int)nvol, label, hsize);
+  const bool direct = pl.direct_writes;
+  // direct writes and no minimum IDs: k_label writes label_out itself and
+  // k_out_init is not needed
+  const bool fused = direct && !r.d_minid;
+  k_label_init<<<tiles_for(N), 256, 0, st>>>(NN, dV, dVB, (int)nvol, fused ? nullptr : label,
+                                             fused ? nullptr : hsize, fused ? r.d_label : nullptr,
+                                             fused ? r.d_next : nullptr);
   FOF_HIP(hipGetLastError());
   if ((rc = te(9))) return rc;
   if ((rc = tb(8))) return rc;
-  const bool direct = pl.direct_writes;
   const uint32_t glab = (uint32_t)((N + kLabTile - 1) / kLabTile);
   if (direct)
     k_label<true><<<glab, 256, 0, st>>>(NN, par, pos4, minorig, gsize, label, hsize, cursor, sk0,
                                         sv0, dbase, stride, offid, idroot, B.idhead, single, dV,
-                                        dVB, (int)nvol);
+                                        dVB, (int)nvol, fused ? r.d_label : nullptr);
   else
     k_label<false><<<glab, 256, 0, st>>>(NN, par, pos4, minorig, gsize, label, hsize, cursor, sk0,
                                          sv0, dbase, stride, offid, idroot, B.idhead, single, dV,
-                                         dVB, (int)nvol);
+                                         dVB, (int)nvol, nullptr);
   FOF_HIP(hipGetLastError());
   if (!direct && (rc = scatter_u32(sv0, sk0, p32, p32 + N, N, small + kSmM, N, label, kFlag,
                                    B.tmp, st)))
     return rc;
-  k_out_init<<<tiles_for(N), 256, 0, st>>>(NN, label, dVB, (int)nvol, r.d_label, r.d_next, hsize,
-                                           dbase, stride, offid, B.idhead, r.d_minid);
-  FOF_HIP(hipGetLastError());
+  if (!fused) {
+    k_out_init<<<tiles_for(N), 256, 0, st>>>(NN, label, dVB, (int)nvol, r.d_label, r.d_next,
+                                             hsize, dbase, stride, offid, B.idhead, r.d_minid);
+    FOF_HIP(hipGetLastError());
+  }
   if ((rc = te(8))) return rc;
   // members in index order inside each group (large groups through the
   // free cell-sorted positions)

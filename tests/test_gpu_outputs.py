@@ -220,6 +220,24 @@ def test_lattice_at_linking_length(ctx, below):
         assert st.rechecks > 0
 
 
+@pytest.mark.parametrize("side", [12, 18, 28])
+def test_group_sizes_across_sort_paths(ctx, side):
+    # one lattice group (a = l) of 1728, 5832 or 21952 members, beside a
+    # randomly permuted background of pairs and small groups: every segment
+    # sort path runs (lanes, one wave, a workgroup in LDS, the radix sort)
+    a = 0.25
+    g = np.arange(side) * a + 1.0
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(side)
+    far = rng.random((500, 3)) * 5.5 + 9.5
+    pairs = np.concatenate([far, far + np.array([0.1, 0.0, 0.0])])
+    pos = np.concatenate([lat, pairs])[rng.permutation(len(lat) + len(pairs))]
+    P = records(pos, side)
+    dom = L.domain([0] * 3, [16] * 3, [0] * 3, [16] * 3, [0.75] * 3, a, 2)
+    olab, st = run_edge(ctx, P, dom)
+    assert np.bincount(np.unique(olab, return_inverse=True)[1]).max() == side ** 3
+
+
 @pytest.mark.parametrize("fmt", ["ascii", "gadget1", "picola"])
 def test_read_format_ghosts_link(ctx, fmt):
     # a fixture in each of the reference's input formats: read on the host,

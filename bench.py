@@ -176,7 +176,7 @@ B_UF, B_RED = 12.0, 28.0
 PASS_KERNELS = {
     "sort": ["k_pa_count", "k_pa_starts", "k_pb_tiles", "k_pa_scatter", "k_pb_count", "k_pb_scatter",
              "k_pb_starts", "k_pc_sort", "k_scan_reduce", "k_scan_single", "k_scan_down"],
-    "union_find": ["k_init_uf", "k_flatten", "k_label_init", "k_label", "k_put", "k_out_init"],
+    "union_find": ["k_init_uf", "k_flatten", "k_label", "k_put", "k_out_init"],
     "halo_sums": ["k_halo_sums"],
 }
 

@@ -1584,7 +1584,6 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
       return (uint32_t)(v & ((1ull << len) - 1));
     };
     const bool bmap = na > kMapRuns;
-    constexpr uint32_t kWalks = 2;
     // bits of x in [c - 1, c + 1] for a row starting at x0, given s = c - 1 - x0
     auto touchbits = [](int s) -> uint32_t { return s >= 0 ? 7u << s : 7u >> -s; };
     // a run pair (LDS slots sa, sb with heads ha, hb) not yet joined: small
@@ -1665,21 +1664,30 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
     // prune the later rounds (a full queue falls back to testing in place)
     if (bmap) {
       // items (row of the forward window, run), row-major, rows centre-out;
-      // the first walk (touching sub-cells) before the second
-      const uint32_t nrow = (uint32_t)((reach[2] + 1) * wy);
-      const uint32_t half = nrow * (uint32_t)na;
-      auto aa_items = [&](uint32_t from, uint32_t to) {
-      for (uint32_t base = from; base < to; base += ROUND) {
+      // the first walk (touching sub-cells: the 5 forward rows at |dy|, dz
+      // <= 1) before the second (every forward row)
+      const int ry = reach[1], rz = reach[2];
+      const uint32_t nrow1 = 5, nrow2 = (uint32_t)(ry + 1 + rz * wy);
+      auto aa_items = [&](bool first) {
+      const uint32_t to = (first ? nrow1 : nrow2) * (uint32_t)na;
+      for (uint32_t base = 0; base < to; base += ROUND) {
         const uint32_t pe = min(to, base + (uint32_t)ROUND);
         const bool masked = mainr != kNone;
         for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
           const uint32_t r2 = p / (uint32_t)na, ia = p - r2 * (uint32_t)na;
-          const bool first = kWalks == 2 && r2 < nrow;  // touching sub-cells only
-          const uint32_t ri = (kWalks == 2 && !first) ? r2 - nrow : r2;
-          const int dz = (int)(ri / (uint32_t)wy), dy = zig((int)(ri % (uint32_t)wy));
-          if (dz == 0 && dy < 0) continue;  // not forward
-          const bool trow = kWalks == 2 && dz <= 1 && dy >= -1 && dy <= 1;
-          if (first && !trow) continue;
+          int dz, dy;
+          if (first) {
+            dz = r2 < 2 ? 0 : 1;
+            dy = zig((int)(r2 < 2 ? r2 : r2 - 2));
+          } else if (r2 <= (uint32_t)ry) {
+            dz = 0;
+            dy = (int)r2;
+          } else {
+            const uint32_t t = r2 - (uint32_t)(ry + 1);
+            dz = 1 + (int)(t / (uint32_t)wy);
+            dy = zig((int)(t % (uint32_t)wy));
+          }
+          const bool trow = dz <= 1 && dy >= -1 && dy <= 1;
           const uint32_t ra = arel[ia];
           const int ax = ra & 15, ay = (ra >> 4) & 15, az = ra >> 8;
           const int by = ay + dy, bz = az + dz;
@@ -1707,7 +1715,7 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
         drain();
       }
       };
-      aa_items(0, half);
+      aa_items(true);
       // the dominant set: a vote of 64 runs spread over the cell (wave 0);
       // kept when it holds at least a quarter of them
       if (threadIdx.x < 64) {
@@ -1733,7 +1741,7 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
           }
       }
       __syncthreads();
-      aa_items(half, 2 * half);
+      aa_items(false);
     } else {
       const uint32_t total = (uint32_t)na * (uint32_t)na;  // na <= kMapRuns
       for (uint32_t base = 0; base < total; base += ROUND) {
@@ -1794,19 +1802,20 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
       const uint32_t nb = cnt;
       // every (home run, neighbour item) pair within reach, in rounds
       if (bmap) {
-        // items (row of the window, item), row-major, rows centre-out
+        // items (row of the window, item), row-major, rows centre-out: the
+        // 9 touching rows (touching sub-cells) first, then every row
         const uint32_t nrow = (uint32_t)((2 * reach[2] + 1) * wy);
-        const uint32_t total = kWalks * nrow * nb;
+        const uint32_t total = (9 + nrow) * nb;
         for (uint32_t base = 0; base < total; base += ROUND) {
           const uint32_t pe = min(total, base + (uint32_t)ROUND);
           const uint32_t mr = mainr;
           for (uint32_t p = base + threadIdx.x; p < pe; p += blockDim.x) {
             const uint32_t r2 = p / nb, jb = p - r2 * nb;
-            const bool first = kWalks == 2 && r2 < nrow;
-            const uint32_t ri = (kWalks == 2 && !first) ? r2 - nrow : r2;
-            const int dz = zig((int)(ri / (uint32_t)wy)), dy = zig((int)(ri % (uint32_t)wy));
-            const bool trow = kWalks == 2 && dz >= -1 && dz <= 1 && dy >= -1 && dy <= 1;
-            if (first && !trow) continue;
+            const bool first = r2 < 9;
+            const uint32_t ri = first ? r2 : r2 - 9;
+            const uint32_t rw = first ? 3u : (uint32_t)wy;
+            const int dz = zig((int)(ri / rw)), dy = zig((int)(ri % rw));
+            const bool trow = dz >= -1 && dz <= 1 && dy >= -1 && dy <= 1;
             const uint32_t rb = brel[jb];
             const int bx = rel_x(rb), ay = rel_y(rb) + dy, az = rel_z(rb) + dz;
             if (az < 0 || az >= mm[2] || ay < 0 || ay >= mm[1] || !gap_ok(0, dy, dz)) continue;

@@ -1900,8 +1900,9 @@ constexpr int kSmallQ = 128;    // neighbour candidates queued in LDS (< 64 + 64
 constexpr int kSmallWaves = 4;  // waves per workgroup
 constexpr float kFar = 1e18f;   // position of an absent particle: links nothing
 
+constexpr int kBlk = 8;  // partners per block of a wave's pair tests
 struct SmallLds {
-  float4 apos[kBigCell];             // the cell's particles
+  float4 apos[kBigCell + kBlk];      // the cell's particles, then kBlk absent ones
   unsigned long long cm[4][4];       // the four largest components (bit masks)
   uint32_t lp[kBigCell];             // union-find over the cell's components
   float4 qpos[kSmallQ];   // queued neighbour candidates
@@ -1917,29 +1918,26 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Partners per test block (scalar registers: 4 per partner).
-constexpr int kBlk = 8;
-
 // The lane's particle (xi, yi, zi) against the kBlk partners p[0..kBlk-1]
 // in LDS, the same for every lane (broadcast reads, all issued before the
-// tests); partners k >= jn are absent (placed at -kFar, absent lanes sit at
-// +kFar, so nothing links them).  Returns bit k = linked under the reference's test: the FP32
+// tests).  Past the cell's last particle the table holds kBlk absent ones at
+// -kFar (absent lanes sit at +kFar), so a short last block needs no masking:
+// nothing links them.  Returns bit k = linked under the reference's test: the FP32
 // decision outside the band; when any lane has a pair inside the band, the
 // block is re-decided pair by pair with pair_linked.  With COLS, cols[k] is
 // the lane mask of partner k's links (the transposed block).
 template <bool COLS>
 __device__ __forceinline__ uint32_t testblk(float xi, float yi, float zi,
-                                           const float4 *__restrict__ p, uint32_t jn,
-                                           const Rule &R, uint32_t &rechecks,
+                                           const float4 *__restrict__ p, const Rule &R,
+                                           uint32_t &rechecks,
                                            unsigned long long (&cols)[kBlk]) {
   uint32_t lk = 0;
   unsigned long long band = 0;  // lanes with an undecided pair (uniform)
   float qx[kBlk], qy[kBlk], qz[kBlk];
 #pragma unroll
   for (int k = 0; k < kBlk; k++) {
-    const float4 q0 = p[min((uint32_t)k, jn - 1)];
-    const bool in = (uint32_t)k < jn;
-    qx[k] = in ? q0.x : -kFar, qy[k] = in ? q0.y : -kFar, qz[k] = in ? q0.z : -kFar;
+    const float4 q0 = p[k];
+    qx[k] = q0.x, qy[k] = q0.y, qz[k] = q0.z;
   }
 #pragma unroll
   for (int k = 0; k < kBlk; k++) {
@@ -1993,8 +1991,8 @@ __device__ __forceinline__ void small_cell_own(SmallLds &S, uint32_t nA, const R
       const uint32_t jn = min(64u, nA - 64 * tb);
       for (uint32_t j0 = 0; j0 < jn; j0 += kBlk) {
         unsigned long long cols[kBlk];
-        const uint32_t h = testblk<true>(xr[ta], yr[ta], zr[ta], pa + 64 * tb + j0, jn - j0, R,
-                                         rechecks, cols);
+        const uint32_t h = testblk<true>(xr[ta], yr[ta], zr[ta], pa + 64 * tb + j0, R, rechecks,
+                                         cols);
         rows[ta][tb] |= (unsigned long long)h << j0;
         if (tb != ta) {  // the transposed block: rows of 64 tb + j0 + k, word ta
 #pragma unroll
@@ -2168,8 +2166,8 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
       unsigned long long mw = 0;
       for (uint32_t j0 = 0; j0 < jn; j0 += kBlk) {
         unsigned long long cols[kBlk];
-        mw |= (unsigned long long)testblk<false>(bx, by, bz, S.apos + 64 * w + j0, jn - j0, R,
-                                                 rechecks, cols)
+        mw |= (unsigned long long)testblk<false>(bx, by, bz, S.apos + 64 * w + j0, R, rechecks,
+                                                 cols)
               << j0;
       }
       m[w] = vb ? mw : 0ull;
@@ -2264,7 +2262,9 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
   wave_sync();
 }
 
-__global__ __launch_bounds__(64 * kSmallWaves) void k_link_small(
+// Four waves per SIMD (at most 128 VGPRs): left to itself the compiler
+// takes 134 for the hoisted partner loads, which allows three (12% slower).
+__global__ __launch_bounds__(64 * kSmallWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_link_small(
     const uint32_t *__restrict__ dense, const uint32_t *__restrict__ ndense,
     const float4 *__restrict__ pos4, const Grid G, uint32_t T, uint32_t *__restrict__ par,
     const VolDev *__restrict__ vols, const uint32_t *__restrict__ vbeg, int nvol,
@@ -2291,6 +2291,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_link_small(
     const VolDev &V = vols[vol_of(vbeg, nvol, sA)];
     const float4 p0 = pos4[sA];
     for (uint32_t i = lane; i < nA; i += 64) S.apos[i] = ld4(pos4 + sA + i);
+    if (lane < (uint32_t)kBlk) S.apos[nA + lane] = make_float4(-kFar, -kFar, -kFar, 0.f);
     const CellPos cp = cell_pos(V, p0.x, p0.y, p0.z);
     const Rule R = make_rule(V, cp.ix, cp.iy, cp.iz);
     // neighbour k of the table (lane k): its particle range

@@ -1900,7 +1900,7 @@ constexpr int kSmallQ = 128;    // neighbour candidates queued in LDS (< 64 + 64
 constexpr int kSmallWaves = 4;  // waves per workgroup
 constexpr float kFar = 1e18f;   // position of an absent particle: links nothing
 
-constexpr int kBlk = 8;  // partners per block of a wave's pair tests
+constexpr int kBlk = 4;  // partners per block of a wave's pair tests
 struct SmallLds {
   float4 apos[kBigCell + kBlk];      // the cell's particles, then kBlk absent ones
   unsigned long long cm[4][4];       // the four largest components (bit masks)
@@ -2262,9 +2262,11 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
   wave_sync();
 }
 
-// Four waves per SIMD (at most 128 VGPRs): left to itself the compiler
-// takes 134 for the hoisted partner loads, which allows three (12% slower).
-__global__ __launch_bounds__(64 * kSmallWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_link_small(
+// Five waves per SIMD, as many as the LDS tables allow (at most 96 VGPRs, a
+// few spilled).  On the clumpy 1024^3 box: 33.4 ms with blocks of 8 partners
+// and no cap (134 VGPRs, three waves), 28.3 ms at four waves, 25.0 ms with
+// blocks of 4 at five.
+__global__ __launch_bounds__(64 * kSmallWaves) __attribute__((amdgpu_waves_per_eu(5))) void k_link_small(
     const uint32_t *__restrict__ dense, const uint32_t *__restrict__ ndense,
     const float4 *__restrict__ pos4, const Grid G, uint32_t T, uint32_t *__restrict__ par,
     const VolDev *__restrict__ vols, const uint32_t *__restrict__ vbeg, int nvol,

@@ -1909,10 +1909,13 @@ constexpr float kFar = 1e18f;   // position of an absent particle: links nothing
 
 constexpr int kBlk = 4;  // partners per block of a wave's pair tests
 struct SmallLds {
-  float4 apos[kBigCell + kBlk];      // the cell's particles, then kBlk absent ones
+  // the cell's particles, x, y and z apart (the w field is not needed; the
+  // float4 table cost 6 KB more per workgroup and 6% of the kernel's time),
+  // then kBlk absent ones
+  float ax[kBigCell + kBlk], ay[kBigCell + kBlk], az[kBigCell + kBlk];
   unsigned long long cm[4][4];       // the four largest components (bit masks)
   uint32_t lp[kBigCell];             // union-find over the cell's components
-  float4 qpos[kSmallQ];   // queued neighbour candidates
+  float qx[kSmallQ], qy[kSmallQ], qz[kSmallQ];  // queued neighbour candidates
   uint32_t qidx[kSmallQ];
 };
 
@@ -1935,17 +1938,16 @@ __device__ __forceinline__ void wave_sync() {
 // the lane mask of partner k's links (the transposed block).
 template <bool COLS>
 __device__ __forceinline__ uint32_t testblk(float xi, float yi, float zi,
-                                           const float4 *__restrict__ p, const Rule &R,
+                                           const float *__restrict__ px,
+                                           const float *__restrict__ py,
+                                           const float *__restrict__ pz, const Rule &R,
                                            uint32_t &rechecks,
                                            unsigned long long (&cols)[kBlk]) {
   uint32_t lk = 0;
   unsigned long long band = 0;  // lanes with an undecided pair (uniform)
   float qx[kBlk], qy[kBlk], qz[kBlk];
 #pragma unroll
-  for (int k = 0; k < kBlk; k++) {
-    const float4 q0 = p[k];
-    qx[k] = q0.x, qy[k] = q0.y, qz[k] = q0.z;
-  }
+  for (int k = 0; k < kBlk; k++) qx[k] = px[k], qy[k] = py[k], qz[k] = pz[k];
 #pragma unroll
   for (int k = 0; k < kBlk; k++) {
     const float dx = xi - qx[k], dy = yi - qy[k], dz = zi - qz[k];
@@ -1970,21 +1972,21 @@ __device__ __forceinline__ uint32_t testblk(float xi, float yi, float zi,
 }
 
 // The cell's own pairs (NW 64-particle tiles, 64 (NW - 1) < nA <= 64 NW;
-// the particles in S.apos): the link matrix, its components (breadth-first,
+// the particles in S.ax, S.ay, S.az): the link matrix, its components (breadth-first,
 // by pulling: a particle joins when its row meets the frontier), each
 // particle's component root (the lowest member) into lp, and the four
 // largest components' masks into S.cm.
 template <int NW>
 __device__ __forceinline__ void small_cell_own(SmallLds &S, uint32_t nA, const Rule &R,
                                                uint32_t &rechecks) {
-  const float4 *const pa = S.apos;
   const uint32_t lane = threadIdx.x & 63;
   float xr[NW], yr[NW], zr[NW];
 #pragma unroll
   for (int r = 0; r < NW; r++) {
     const uint32_t i = 64 * r + lane;
-    const float4 q = pa[i < nA ? i : 0];
-    xr[r] = i < nA ? q.x : kFar, yr[r] = i < nA ? q.y : kFar, zr[r] = i < nA ? q.z : kFar;
+    const uint32_t ic = i < nA ? i : 0;
+    xr[r] = i < nA ? S.ax[ic] : kFar, yr[r] = i < nA ? S.ay[ic] : kFar;
+    zr[r] = i < nA ? S.az[ic] : kFar;
   }
   unsigned long long rows[NW][NW];  // rows[r][w]: links of 64 r + lane to 64 w ..
 #pragma unroll
@@ -1998,8 +2000,9 @@ __device__ __forceinline__ void small_cell_own(SmallLds &S, uint32_t nA, const R
       const uint32_t jn = min(64u, nA - 64 * tb);
       for (uint32_t j0 = 0; j0 < jn; j0 += kBlk) {
         unsigned long long cols[kBlk];
-        const uint32_t h = testblk<true>(xr[ta], yr[ta], zr[ta], pa + 64 * tb + j0, R, rechecks,
-                                         cols);
+        const uint32_t o = 64 * tb + j0;
+        const uint32_t h =
+            testblk<true>(xr[ta], yr[ta], zr[ta], S.ax + o, S.ay + o, S.az + o, R, rechecks, cols);
         rows[ta][tb] |= (unsigned long long)h << j0;
         if (tb != ta) {  // the transposed block: rows of 64 tb + j0 + k, word ta
 #pragma unroll
@@ -2126,9 +2129,9 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
     for (int r = 0; r < NW; r++) {
       const uint32_t i = 64 * r + lane;
       if (i < nA) {
-        const float4 q = S.apos[i];
-        mn[0] = fminf(mn[0], q.x), mn[1] = fminf(mn[1], q.y), mn[2] = fminf(mn[2], q.z);
-        mx[0] = fmaxf(mx[0], q.x), mx[1] = fmaxf(mx[1], q.y), mx[2] = fmaxf(mx[2], q.z);
+        const float qx = S.ax[i], qy = S.ay[i], qz = S.az[i];
+        mn[0] = fminf(mn[0], qx), mn[1] = fminf(mn[1], qy), mn[2] = fminf(mn[2], qz);
+        mx[0] = fmaxf(mx[0], qx), mx[1] = fmaxf(mx[1], qy), mx[2] = fmaxf(mx[2], qz);
       }
     }
 #pragma unroll
@@ -2162,9 +2165,9 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
   auto flush = [&](uint32_t cnt) {
     wave_sync();
     const bool vb = lane < cnt;
-    const float4 pb = S.qpos[vb ? lane : 0];
-    const uint32_t jb = S.qidx[vb ? lane : 0];
-    const float bx = vb ? pb.x : kFar, by = vb ? pb.y : kFar, bz = vb ? pb.z : kFar;
+    const uint32_t lq = vb ? lane : 0;
+    const uint32_t jb = S.qidx[lq];
+    const float bx = vb ? S.qx[lq] : kFar, by = vb ? S.qy[lq] : kFar, bz = vb ? S.qz[lq] : kFar;
     tests += vb ? nA : 0u;
     unsigned long long m[NW];
 #pragma unroll
@@ -2173,8 +2176,9 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
       unsigned long long mw = 0;
       for (uint32_t j0 = 0; j0 < jn; j0 += kBlk) {
         unsigned long long cols[kBlk];
-        mw |= (unsigned long long)testblk<false>(bx, by, bz, S.apos + 64 * w + j0, R, rechecks,
-                                                 cols)
+        const uint32_t o = 64 * w + j0;
+        mw |= (unsigned long long)testblk<false>(bx, by, bz, S.ax + o, S.ay + o, S.az + o, R,
+                                                 rechecks, cols)
               << j0;
       }
       m[w] = vb ? mw : 0ull;
@@ -2231,7 +2235,7 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
     const unsigned long long bal = __ballot(keep);
     if (keep) {
       const uint32_t at = qn + (uint32_t)__popcll(bal & ((1ull << lane) - 1));
-      S.qpos[at] = pb;
+      S.qx[at] = pb.x, S.qy[at] = pb.y, S.qz[at] = pb.z;
       S.qidx[at] = j;
     }
     qn += (uint32_t)__popcll(bal);
@@ -2239,10 +2243,11 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
       flush(64);
       // move the overflow down (qn - 64 < 64 entries)
       const bool mv = lane < qn - 64;
-      const float4 t = S.qpos[64 + (mv ? lane : 0)];
-      const uint32_t ti = S.qidx[64 + (mv ? lane : 0)];
+      const uint32_t lm = 64 + (mv ? lane : 0);
+      const float tx = S.qx[lm], ty = S.qy[lm], tz = S.qz[lm];
+      const uint32_t ti = S.qidx[lm];
       wave_sync();
-      if (mv) S.qpos[lane] = t, S.qidx[lane] = ti;
+      if (mv) S.qx[lane] = tx, S.qy[lane] = ty, S.qz[lane] = tz, S.qidx[lane] = ti;
       qn -= 64;
     }
   };
@@ -2269,10 +2274,10 @@ __device__ __forceinline__ void small_cell(SmallLds &S, uint32_t sA, uint32_t nA
   wave_sync();
 }
 
-// Five waves per SIMD, as many as the LDS tables allow (at most 96 VGPRs, a
-// few spilled).  On the clumpy 1024^3 box: 33.4 ms with blocks of 8 partners
-// and no cap (134 VGPRs, three waves), 28.3 ms at four waves, 25.0 ms with
-// blocks of 4 at five.
+// Five waves per SIMD (at most 96 VGPRs, a few spilled).  On the clumpy
+// 1024^3 box: 33.4 ms with blocks of 8 partners and no cap (134 VGPRs, three
+// waves), 28.3 ms at four waves, 25.0 ms with blocks of 4 at five, 23.5 ms
+// with the x, y, z tables; six waves (80 VGPRs) spill enough to take 31.4 ms.
 __global__ __launch_bounds__(64 * kSmallWaves) __attribute__((amdgpu_waves_per_eu(5))) void k_link_small(
     const uint32_t *__restrict__ dense, const uint32_t *__restrict__ ndense,
     const float4 *__restrict__ pos4, const Grid G, uint32_t T, uint32_t *__restrict__ par,
@@ -2299,8 +2304,11 @@ __global__ __launch_bounds__(64 * kSmallWaves) __attribute__((amdgpu_waves_per_e
     const uint32_t cN = (d + ws < nd) ? dense[d + ws] : 0u;
     const VolDev &V = vols[vol_of(vbeg, nvol, sA)];
     const float4 p0 = pos4[sA];
-    for (uint32_t i = lane; i < nA; i += 64) S.apos[i] = ld4(pos4 + sA + i);
-    if (lane < (uint32_t)kBlk) S.apos[nA + lane] = make_float4(-kFar, -kFar, -kFar, 0.f);
+    for (uint32_t i = lane; i < nA; i += 64) {
+      const float4 q = ld4(pos4 + sA + i);
+      S.ax[i] = q.x, S.ay[i] = q.y, S.az[i] = q.z;
+    }
+    if (lane < (uint32_t)kBlk) S.ax[nA + lane] = -kFar, S.ay[nA + lane] = -kFar, S.az[nA + lane] = -kFar;
     const CellPos cp = cell_pos(V, p0.x, p0.y, p0.z);
     const Rule R = make_rule(V, cp.ix, cp.iy, cp.iz);
     // neighbour k of the table (lane k): its particle range

@@ -1480,7 +1480,6 @@ __global__ __launch_bounds__(NT, WPE) void k_link_dense(const uint32_t *__restri
   __shared__ uint32_t nbS[26], nbOff[27];
   __shared__ uint8_t nbDense[26];
   __shared__ uint32_t cnt, qn, dnext;
-  // one-wave cells: run sizes in LDS (home runs fit a byte: < 256 particles)
   uint32_t tests = 0, rechecks = 0, joined = 0;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nd = *ndense;

@@ -293,6 +293,126 @@ int fof_write_halos(const char *path, uint32_t flags, const fof_halo_v1 *halos,
   return ferror(f) ? FOF_E_ARG : 0;
 }
 
+// src_v2/FOF.c:172-284.  The visited flags play the part of the reference's
+// head[] (every member but the first-found one is marked by the counting
+// walk, :192-200), so groups are taken in ascending order of their lowest
+// index.
+int fof_subsample(const fof_particles_v1 *P, const uint32_t *next, const fof_subsample_v1 *s,
+                  fof_uniform_fn uniform, void *state, uint32_t *inhalo, uint64_t in_cap,
+                  uint64_t *n_in, uint32_t *outhalo, uint64_t out_cap, uint64_t *n_out) {
+  if (!P || !s || !n_in || !n_out) return FOF_E_ARG;
+  *n_in = 0, *n_out = 0;
+  if (P->version != FOF_ABI_VERSION || P->location != FOF_MEM_HOST) return FOF_E_ARG;
+  if (P->n > 0xFFFFFFFEull) return FOF_E_RANGE;
+  const uint32_t n = (uint32_t)P->n;
+  if (n && (!P->base || !next)) return FOF_E_ARG;
+  const bool sin = s->samp_in > 0, sout = s->samp_out > 0;
+  if ((sin || sout) && !uniform) return FOF_E_ARG;
+  if ((sin && !inhalo && in_cap) || (sout && !outhalo && out_cap)) return FOF_E_ARG;
+  for (uint32_t i = 0; i < n; i++)
+    if (next[i] < 1 || next[i] > n) return FOF_E_ARG;
+  const uint8_t *base = static_cast<const uint8_t *>(P->base);
+  // Pos is float, the box double: the comparisons promote (src_v2/FOF.c:209-231)
+  auto outside = [&](uint32_t ip) {
+    const uint8_t *r = base + (uint64_t)(ip - 1) * P->stride + P->off_pos;
+    for (int a = 0; a < 3; a++) {
+      float x;
+      memcpy(&x, r + 4 * a, 4);
+      if ((double)x < s->rmin[a]) return true;
+      if (((s->closed_upper >> a) & 1u) ? ((double)x > s->rmax[a]) : ((double)x >= s->rmax[a]))
+        return true;
+    }
+    return false;
+  };
+  std::vector<uint8_t> seen((size_t)n + 1, 0);
+  uint64_t ni = 0, no = 0;
+  int rc = 0;
+  for (uint32_t i = 1; i <= n && !rc; i++) {
+    if (seen[i]) continue;
+    uint32_t ip = i;
+    uint64_t np = 0;
+    for (;;) {  // the group's size
+      ip = next[ip - 1];
+      np++;
+      if (ip == i) break;
+      if (seen[ip] || np >= n) return FOF_E_ARG;  // not a ring through i
+      seen[ip] = 1;
+    }
+    const bool halo = !((double)np < s->nphalomin);  // src_v2/FOF.c:202
+    const double rate = halo ? s->samp_in : s->samp_out;
+    if (!(rate > 0)) continue;
+    uint32_t *list = halo ? inhalo : outhalo;
+    uint64_t &cnt = halo ? ni : no;
+    const uint64_t cap = halo ? in_cap : out_cap;
+    ip = i;
+    for (;;) {
+      ip = next[ip - 1];
+      if (uniform(state) < rate) {
+        if (outside(ip)) break;
+        if (cnt >= cap) {
+          rc = FOF_E_RANGE;
+          break;
+        }
+        list[cnt++] = ip;
+      }
+      if (ip == i) break;
+    }
+  }
+  *n_in = ni, *n_out = no;
+  return rc;
+}
+
+// src_v2/FOF.c:432-544
+int fof_write_subsample(const char *path, uint32_t flags, const fof_particles_v1 *P,
+                        const uint32_t *idx, uint64_t n) {
+  if (!path || !P || (n && !idx)) return FOF_E_ARG;
+  if (P->version != FOF_ABI_VERSION || P->location != FOF_MEM_HOST || (P->n && !P->base))
+    return FOF_E_ARG;
+  const bool unf = (flags & FOF_OUT_UNFORMATTED) != 0, withid = (flags & FOF_OUT_PARTICLE_ID) != 0;
+  if (withid && P->off_id < 0) return FOF_E_ARG;
+  for (uint64_t k = 0; k < n; k++)
+    if (idx[k] < 1 || idx[k] > P->n) return FOF_E_ARG;
+  if (n == 0) return 0;  // nsubsamp > 0 guards the file (src_v2/FOF.c:457)
+  File F(path, unf ? "wb" : "w");
+  if (!F.f) return FOF_E_ARG;
+  FILE *f = F.f;
+  const uint8_t *base = static_cast<const uint8_t *>(P->base);
+  auto rec = [&](uint64_t k) { return base + (uint64_t)(idx[k] - 1) * P->stride; };
+  if (unf) {
+    // the reference's record lengths are int products (truncated like them)
+    const uint32_t cnt = (uint32_t)n;
+    put_rec(f, 4);
+    fwrite(&cnt, 4, 1, f);
+    put_rec(f, 4);
+    for (int part = 0; part < 2; part++) {  // all positions, then all velocities
+      const uint32_t off = part ? P->off_vel : P->off_pos;
+      put_rec(f, (uint32_t)(12u * n));
+      for (uint64_t k = 0; k < n; k++) fwrite(rec(k) + off, 4, 3, f);
+      put_rec(f, (uint32_t)(12u * n));
+    }
+    if (withid) {
+      put_rec(f, (uint32_t)(8u * n));
+      for (uint64_t k = 0; k < n; k++) fwrite(rec(k) + P->off_id, 8, 1, f);
+      put_rec(f, (uint32_t)(8u * n));
+    }
+  } else {
+    for (uint64_t k = 0; k < n; k++) {
+      float v[6];
+      memcpy(v, rec(k) + P->off_pos, 12);
+      memcpy(v + 3, rec(k) + P->off_vel, 12);
+      if (withid) {
+        unsigned long long id;
+        memcpy(&id, rec(k) + P->off_id, 8);
+        fprintf(f, "%15llu %15.6f %15.6f %15.6f %15.6f %15.6f %15.6f\n", id, v[0], v[1], v[2], v[3],
+                v[4], v[5]);
+      } else {
+        fprintf(f, "%15.6f %15.6f %15.6f %15.6f %15.6f %15.6f\n", v[0], v[1], v[2], v[3], v[4], v[5]);
+      }
+    }
+  }
+  return ferror(f) ? FOF_E_ARG : 0;
+}
+
 // src_v2/FOF.c:917-957
 int fof_write_info(const char *path, const double lmin[3], const double lmax[3], int nx, int ny,
                    int nz, const int32_t *nhalos, const uint32_t *ninhalo,

@@ -11,7 +11,7 @@ from . import _lib as L
 
 # Every function fof_io.h declares.
 IO_SYMBOLS = ("fof_read_ascii", "fof_read_gadget1", "fof_read_picola", "fof_write_halos",
-              "fof_write_info")
+              "fof_write_info", "fof_subsample", "fof_write_subsample")
 
 OUT_UNFORMATTED, OUT_PARTICLES, OUT_INERTIA, OUT_DISPERSION, OUT_PARTICLE_ID = 0x1, 0x2, 0x4, 0x8, 0x10
 
@@ -40,6 +40,22 @@ class GadgetHeader(C.Structure):
 
 assert C.sizeof(GadgetHeader) == 256
 
+
+class Subsample(C.Structure):
+    """fof_subsample_v1"""
+    _fields_ = [
+        ("samp_in", C.c_double),
+        ("samp_out", C.c_double),
+        ("rmin", C.c_double * 3),
+        ("rmax", C.c_double * 3),
+        ("nphalomin", C.c_double),
+        ("closed_upper", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+UNIFORM_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
+
 _bound = False
 
 
@@ -59,6 +75,11 @@ def _lib():
         lib.fof_write_info.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                        C.c_int, C.c_int, vp, vp, vp]
         lib.fof_write_info.restype = C.c_int
+        lib.fof_subsample.argtypes = [C.POINTER(L.Particles), vp, C.POINTER(Subsample), UNIFORM_FN, vp,
+                                      vp, u64, C.POINTER(C.c_uint64), vp, u64, C.POINTER(C.c_uint64)]
+        lib.fof_subsample.restype = C.c_int
+        lib.fof_write_subsample.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(L.Particles), vp, u64]
+        lib.fof_write_subsample.restype = C.c_int
         _bound = True
     return lib
 
@@ -123,3 +144,32 @@ def write_info(path, lmin, lmax, nx, ny, nz, nhalos, ninhalo=None, nouthalo=None
     return _lib().fof_write_info(str(path).encode(), lo, hi, nx, ny, nz, nh.ctypes.data,
                                  None if ni is None else ni.ctypes.data,
                                  None if no is None else no.ctypes.data)
+
+
+def subsample(P, next_out, samp_in, samp_out, rmin, rmax, nphalomin, uniform, closed_upper=0,
+              in_cap=None, out_cap=None):
+    """Subsampling (src_v2/FOF.c:172-284) of the host records P over the
+    1-based ring next_out; uniform() is called once per draw.  Returns
+    (rc, inhalo, outhalo) with 1-based indices in visit order."""
+    parts = L.particles(P)
+    nxt = np.ascontiguousarray(next_out, dtype=np.uint32)
+    n = len(nxt)
+    s = Subsample(samp_in, samp_out, (C.c_double * 3)(*rmin), (C.c_double * 3)(*rmax), nphalomin,
+                  closed_upper, 0)
+    ic = n if in_cap is None else in_cap
+    oc = n if out_cap is None else out_cap
+    ih = np.zeros(max(ic, 1), np.uint32)
+    oh = np.zeros(max(oc, 1), np.uint32)
+    ni, no = C.c_uint64(), C.c_uint64()
+    fn = UNIFORM_FN(lambda _state: float(uniform()))
+    rc = _lib().fof_subsample(C.byref(parts), nxt.ctypes.data, C.byref(s), fn, None, ih.ctypes.data,
+                              ic, C.byref(ni), oh.ctypes.data, oc, C.byref(no))
+    return rc, ih[:ni.value].copy(), oh[:no.value].copy()
+
+
+def write_subsample(path, flags, P, idx) -> int:
+    """A subsample file (src_v2/FOF.c:432-544) of the 1-based indices idx."""
+    parts = L.particles(P)
+    ix = np.ascontiguousarray(idx, dtype=np.uint32)
+    return _lib().fof_write_subsample(str(path).encode(), flags, C.byref(parts),
+                                      ix.ctypes.data if len(ix) else None, len(ix))

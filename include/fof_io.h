@@ -95,6 +95,51 @@ int fof_write_info(const char *path, const double lmin[3], const double lmax[3],
                    int nz, const int32_t *nhalos, const uint32_t *ninhalo,
                    const uint32_t *nouthalo);
 
+/* Subsampling (src_v2/FOF.c:172-284), on the host after the link: the
+ * groups of a successor ring are visited as the reference's retrieval loop
+ * visits them -- heads in ascending index order, a group of fewer than
+ * nphalomin members "outside halos", any other "inside" -- and when the
+ * matching rate is above zero the group's ring is walked from the head's
+ * successor back to the head with one draw from the caller's generator per
+ * particle visited.  A particle whose draw is below the rate is kept, unless
+ * it lies outside the rank box: that ends the walk of its group (the
+ * reference's `break`, :209-231 and :251-273).  The reference draws from
+ * gsl_rng_uniform on a ranlxd1 generator seeded with Seed (:41-42); passing
+ * that generator here gives the reference's lists for the ring given.
+ * next is the 1-based ring of fof_link_gpu (next[i-1] == the reference's
+ * next[i]) and P the host records; inhalo / outhalo receive the kept
+ * particles' 1-based indices in visit order (each may be NULL when its rate
+ * is not above zero).  FOF_E_RANGE when a list would pass its capacity: the
+ * reference sizes them rint(1.5 n rate), at most n, and stops with a fatal
+ * error once a count reaches that size (:234-238, :276-280).  FOF_E_ARG for
+ * a next[] that is not a set of rings over 1..n. */
+typedef double (*fof_uniform_fn)(void *state);
+
+typedef struct fof_subsample_v1 {
+  double samp_in;        /* SampInHalos  (src_v2/vars.h:106); <= 0: off     */
+  double samp_out;       /* SampOutHalos (src_v2/vars.h:107); <= 0: off     */
+  double rmin[3], rmax[3]; /* the rank box                                   */
+  double nphalomin;      /* src_v2/vars.h:111                               */
+  uint32_t closed_upper; /* bit a: Pos[a] == rmax[a] counts as inside: built
+                            without PERIODIC, on the last rank along axis a
+                            (src_v2/FOF.c:216-229, :258-271)                 */
+  uint32_t reserved;
+} fof_subsample_v1;
+
+int fof_subsample(const fof_particles_v1 *P, const uint32_t *next, const fof_subsample_v1 *s,
+                  fof_uniform_fn uniform, void *state, uint32_t *inhalo, uint64_t in_cap,
+                  uint64_t *n_in, uint32_t *outhalo, uint64_t out_cap, uint64_t *n_out);
+
+/* A subsample file (Subsample, src_v2/FOF.c:432-544; the reference names it
+ * <OutputFileBase>_inhalo.<task> or _outhalo.<task>): the particles idx[0..n)
+ * (1-based, as fof_subsample returns them) of the host records P.  ASCII: one
+ * line per particle, the ID first with FOF_OUT_PARTICLE_ID.  With
+ * FOF_OUT_UNFORMATTED, Fortran records: the count, all positions, all
+ * velocities, then with FOF_OUT_PARTICLE_ID all IDs.  No file is created
+ * when n is 0, as in the reference. */
+int fof_write_subsample(const char *path, uint32_t flags, const fof_particles_v1 *P,
+                        const uint32_t *idx, uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

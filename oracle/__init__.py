@@ -106,8 +106,36 @@ def lib():
         L.or_moments_reference_out.argtypes = [C.c_uint32, f64p, f64p, f64p,
                                                np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS"),
                                                np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
+        L.or_subsample.argtypes = [C.c_uint32, u32p, f32p, C.POINTER(Subsamp), f64p, C.c_uint64,
+                                   C.POINTER(C.c_uint64), u32p, C.POINTER(C.c_uint64), u32p,
+                                   C.POINTER(C.c_uint64)]
+        L.or_subsample.restype = C.c_int
         _lib = L
     return _lib
+
+
+class Subsamp(C.Structure):
+    _fields_ = [("samp_in", C.c_double), ("samp_out", C.c_double), ("rmin", C.c_double * 3),
+                ("rmax", C.c_double * 3), ("nphalomin", C.c_double), ("closed_upper", C.c_uint32)]
+
+
+def subsample(next0, pos, samp_in, samp_out, rmin, rmax, nphalomin, draws, closed_upper=0):
+    """Subsampling (src_v2/FOF.c:172-284) of a 0-based ring with the draws
+    given in order: (inhalo, outhalo, draws used), kept indices 1-based."""
+    nxt = np.ascontiguousarray(next0, dtype=np.uint32)
+    p = _pos(pos)
+    u = np.ascontiguousarray(draws, dtype=np.float64)
+    n = len(nxt)
+    s = Subsamp(samp_in, samp_out, (C.c_double * 3)(*rmin), (C.c_double * 3)(*rmax), nphalomin,
+                closed_upper)
+    ih = np.zeros(max(n, 1), np.uint32)
+    oh = np.zeros(max(n, 1), np.uint32)
+    used, ni, no = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    rc = lib().or_subsample(n, nxt, p, C.byref(s), u, len(u), C.byref(used), ih, C.byref(ni), oh,
+                            C.byref(no))
+    if rc:
+        raise ValueError("oracle subsample: the draws ran out")
+    return ih[:ni.value].copy(), oh[:no.value].copy(), used.value
 
 
 def make_domain(rmin, rmax, rmin_buff, rmax_buff, lcell, linklength, nphalomin) -> Domain:

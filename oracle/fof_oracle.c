@@ -391,6 +391,74 @@ int64_t or_retrieve(uint32_t n, const uint32_t *next0, const float *pos,
   return nh;
 }
 
+/* src_v2/FOF.c:172-284, the PERIODIC and the non-PERIODIC bounds tests
+ * (:209-231, :251-273) selected per axis by closed_upper. */
+static int or_sub_outside(const float *p, const or_subsamp *s) {
+  for (int a = 0; a < 3; a++) {
+    if (p[a] < s->rmin[a]) return 1;
+    if ((s->closed_upper >> a) & 1u) {
+      if (p[a] > s->rmax[a]) return 1;
+    } else {
+      if (p[a] >= s->rmax[a]) return 1;
+    }
+  }
+  return 0;
+}
+
+int or_subsample(uint32_t n, const uint32_t *next0, const float *pos,
+                 const or_subsamp *s, const double *u, uint64_t nu,
+                 uint64_t *used, uint32_t *inhalo, uint64_t *ninhalo,
+                 uint32_t *outhalo, uint64_t *nouthalo) {
+  /* 1-based next[] and head[] as in the reference */
+  uint32_t *next = (uint32_t *)malloc(((size_t)n + 1) * sizeof(uint32_t));
+  unsigned char *head = (unsigned char *)calloc((size_t)n + 1, 1);
+  uint64_t k = 0, ni = 0, no = 0;
+  int rc = 0;
+  for (uint32_t i = 0; i < n; i++) next[i + 1] = next0[i] + 1;
+  for (uint32_t i = 1; i <= n && !rc; i++) {
+    if (head[i] == 0) {
+      uint32_t ip = i;
+      int nphalo = 0;
+      do {
+        ip = next[ip];
+        nphalo++;
+        if (ip == i) break;
+        head[ip] = 1;
+      } while (1);
+      if (nphalo < s->nphalomin) {
+        head[ip] = 1;
+        if (s->samp_out > 0) {
+          do {
+            ip = next[ip];
+            if (k >= nu) { rc = -1; break; }
+            if (u[k++] < s->samp_out) {
+              if (or_sub_outside(pos + 3 * (size_t)(ip - 1), s)) break;
+              outhalo[no++] = ip;
+            }
+            if (ip == i) break;
+          } while (1);
+        }
+      } else {
+        if (s->samp_in > 0) {
+          do {
+            ip = next[ip];
+            if (k >= nu) { rc = -1; break; }
+            if (u[k++] < s->samp_in) {
+              if (or_sub_outside(pos + 3 * (size_t)(ip - 1), s)) break;
+              inhalo[ni++] = ip;
+            }
+            if (ip == i) break;
+          } while (1);
+        }
+      }
+    }
+  }
+  free(next);
+  free(head);
+  *used = k, *ninhalo = ni, *nouthalo = no;
+  return rc;
+}
+
 /* Neumaier summation: s + c carries ~106 bits. */
 static void neu_add(double *s, double *c, double x) {
   double t = *s + x;

@@ -26,6 +26,10 @@
  *                      centred to a few ulps)
  *   or_moments_reference_out  the reference's formulas on hand-checked
  *                      small cases (exact in binary fractions)
+ *   or_subsample       hand-built rings with scripted draws: every member
+ *                      at rate 1, alternate draws, a member outside the box
+ *                      ending its group's walk, the closed upper face, the
+ *                      size cut's < comparison, the number of draws taken
  * The reference itself cannot be compiled here (vars.h includes GSL headers
  * the image lacks), so no golden output of the reference exists.
  */
@@ -109,6 +113,23 @@ void or_moments(uint32_t n, const uint32_t *next0, const float *pos,
 void or_moments_reference_out(uint32_t np, const double *sum_pos,
                               const double *sum_vel, const double *raw,
                               float *inertia, float *disp);
+
+/* Subsampling (src_v2/FOF.c:172-284) over a ring, the draws taken in order
+ * from u[0..nu) (the reference's gsl_rng_uniform calls).  Kept particles go
+ * 1-based, as the reference's ip, to inhalo / outhalo (room for n each);
+ * *used receives the number of draws taken.  Returns -1 when the draws run
+ * out, else 0. */
+typedef struct or_subsamp {
+  double samp_in, samp_out;
+  double rmin[3], rmax[3];
+  double nphalomin;
+  uint32_t closed_upper; /* bit a: the last rank along axis a without PERIODIC */
+} or_subsamp;
+
+int or_subsample(uint32_t n, const uint32_t *next0, const float *pos,
+                 const or_subsamp *s, const double *u, uint64_t nu,
+                 uint64_t *used, uint32_t *inhalo, uint64_t *ninhalo,
+                 uint32_t *outhalo, uint64_t *nouthalo);
 
 #ifdef __cplusplus
 }

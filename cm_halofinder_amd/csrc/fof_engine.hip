@@ -20,8 +20,9 @@
 //                    ("runs")
 //   5 k_link_sparse  one thread per particle of a sparse cell: own cell and
 //                    the 13 forward cells (src_v2/FOF.c:112-153)
-//   6 k_link_dense   dense cells of 256 particles or more, one workgroup
-//                    each: run pairs with early exit
+//   6 k_link_dense   dense cells of 256 particles or more, each linked by
+//                    one workgroup that took it from a queue: run pairs with
+//                    early exit
 //     k_link_small   dense cells below 256 particles, one wave each: every
 //                    pair tested from LDS broadcasts, components by BFS over
 //                    a bit matrix in registers

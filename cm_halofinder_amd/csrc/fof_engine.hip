@@ -4389,7 +4389,7 @@ extern "C" int fof_engine_run(fof_ctx *ctx, const fof_particles_v1 *P, const fof
       if (rc || (rc = trial(best.T, 1, gb, ms))) break;
       if (ms < bms * 0.99) bms = ms, best.gbig = gb;
     }
-    for (uint32_t K : {2u, 4u}) {
+    for (uint32_t K : {2u, 3u, 4u}) {
       if (rc || K > nvol) break;
       if ((rc = trial(best.T, K, best.gbig, ms))) break;
       if (ms < bms * 0.99) bms = ms, best.pipes = K;

@@ -6,8 +6,12 @@
  * and Checkhalo's centre of mass (src_v2/FOF.c:384-401) over the returned
  * next[1..n].  Input: a Gadget-1 file read with the library's reader.
  * Output, one line per owned halo: head (1-based, as ihalo[]), N, CoM.
+ * Then the subsampling walk (src_v2/FOF.c:172-284) through fof_subsample
+ * with the rates given and a 64-bit LCG in place of gsl_rng_uniform: two
+ * lines, "S in <count> <indices>" and "S out <count> <indices>".
  *
  *   fof_caller <gadget1 file> <box> <pside> <linklength> <nphalomin>
+ *              [<SampInHalos> <SampOutHalos> <seed>]
  */
 #include <stddef.h>
 #include <stdint.h>
@@ -29,6 +33,13 @@ static unsigned int nparticles_tot;
 static unsigned int *next;
 static double rmin[3], rmax[3], rmin_buff[3], rmax_buff[3];
 static double linklength, nphalomin;
+
+/* the caller's generator: x <- a x + c (mod 2^64), u = top 53 bits / 2^53 */
+static double lcg_uniform(void *state) {
+  unsigned long long *x = (unsigned long long *)state;
+  *x = *x * 6364136223846793005ull + 1442695040888963407ull;
+  return (double)(*x >> 11) * (1.0 / 9007199254740992.0);
+}
 
 static void FatalError(const char *file, int line) {
   printf("FatalError %s:%d\n", file, line);
@@ -109,6 +120,28 @@ int main(int argc, char **argv) {
         zh >= rmax[2])
       continue;
     printf("%u %d %.17g %.17g %.17g\n", i, np, xh, yh, zh);
+  }
+  if (argc >= 9) {
+    /* ---- subsampling, as INTEGRATION.md binds it ---- */
+    fof_subsample_v1 s = {atof(argv[6]), atof(argv[7]), {rmin[0], rmin[1], rmin[2]},
+                          {rmax[0], rmax[1], rmax[2]}, nphalomin, 0u, 0u};
+    unsigned long long state = strtoull(argv[8], NULL, 10);
+    unsigned int *inhalo = (unsigned int *)malloc(nparticles_tot * sizeof(unsigned int));
+    unsigned int *outhalo = (unsigned int *)malloc(nparticles_tot * sizeof(unsigned int));
+    uint64_t nin = 0, nout = 0;
+    rc = fof_subsample(&fp, next + 1, &s, lcg_uniform, &state, inhalo, nparticles_tot, &nin, outhalo,
+                       nparticles_tot, &nout);
+    if (rc != 0) {
+      printf("\nERROR: fof_subsample: %s\n", fof_strerror(rc));
+      FatalError("FOF.c", 284);
+    }
+    printf("S in %llu", (unsigned long long)nin);
+    for (uint64_t k = 0; k < nin; k++) printf(" %u", inhalo[k]);
+    printf("\nS out %llu", (unsigned long long)nout);
+    for (uint64_t k = 0; k < nout; k++) printf(" %u", outhalo[k]);
+    printf("\n");
+    free(inhalo);
+    free(outhalo);
   }
   free(head0);
   free(next0);

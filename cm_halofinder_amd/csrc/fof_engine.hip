@@ -3803,8 +3803,10 @@ static int enqueue_run(Run &r, hipStream_t st) {
     const uint32_t ra = std::min(maxbins, (uint32_t)kMaxRuns);
     // rounds of 768 candidates: 256 and 384 gave 51.8 and 51.1 ms at 1024^3,
     // 512 and 768 49.5-50.6 and 49.2-49.4, 1024 50.5-50.7, 2048 59.5 (the
-    // 512-entry test queue overflows into in-place tests)
-    k_link_dense<256, 8, kMaxRuns, kRB, 512, 768, 1>
+    // test queue overflows into in-place tests); a queue of 384 entries
+    // 48.6-48.8 against 48.9-49.1 for 256 and 320, 49.1-49.2 for 512 and
+    // 53-54 for 768 or 1024
+    k_link_dense<256, 8, kMaxRuns, kRB, 384, 768, 1>
         <<<gbig, 256, dense_lds(ra, kRB), st>>>(
             dbig, small + kSmNBig, pos4, G, runend, fine, rtoff, rtab, T, par, dV, dVB, (int)nvol,
             ctr, small + kSmOverflow, ra, small + kSmDq);

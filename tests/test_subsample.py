@@ -242,3 +242,32 @@ def test_struct_layout_matches_header(tmp_path):
     S = F.Subsample
     assert got == [ctypes.sizeof(S), S.samp_out.offset, S.rmax.offset, S.nphalomin.offset,
                    S.closed_upper.offset, S.reserved.offset]
+
+
+def test_records_without_id_and_writer_errors(tmp_path):
+    # the 24-byte record (no PARTICLE_ID): Pos at 0, Vel at 12
+    r = records([(10, 0, 12.0)])
+    rec24 = np.zeros(10, np.dtype([("Pos", "<f4", 3), ("Vel", "<f4", 3)]))
+    rec24["Pos"], rec24["Vel"] = r["Pos"], r["Vel"]
+    P24 = L.particles(rec24, stride=24, off_id=-1, off_pos=0, off_vel=12)
+    d = Draws(np.zeros(20))
+    s = F.Subsample(1.0, 1.0, (ctypes.c_double * 3)(*BOX_LO), (ctypes.c_double * 3)(*BOX_HI), 3.0, 0, 0)
+    ih = np.zeros(10, np.uint32)
+    oh = np.zeros(10, np.uint32)
+    ni, no = ctypes.c_uint64(), ctypes.c_uint64()
+    fn = F.UNIFORM_FN(lambda _s: float(d()))
+    lib = F._lib()
+    rc = lib.fof_subsample(ctypes.byref(P24), NEXT_OUT.ctypes.data, ctypes.byref(s), fn, None,
+                           ih.ctypes.data, 10, ctypes.byref(ni), oh.ctypes.data, 10, ctypes.byref(no))
+    assert rc == 0
+    assert ih[:ni.value].tolist() == [4, 8, 1, 6] and oh[:no.value].tolist() == [2, 9, 7]
+    # IDs asked of records without one; an index outside 1..n
+    assert lib.fof_write_subsample(str(tmp_path / "x").encode(), F.OUT_PARTICLE_ID, ctypes.byref(P24),
+                                   ih.ctypes.data, 2) == L.E_ARG
+    bad = np.array([1, 11], np.uint32)
+    assert F.write_subsample(tmp_path / "y", 0, r, bad) == L.E_ARG
+    # the 24-byte records written without IDs
+    assert lib.fof_write_subsample(str(tmp_path / "z").encode(), 0, ctypes.byref(P24), ih.ctypes.data,
+                                   ni.value) == 0
+    vals = np.loadtxt(tmp_path / "z")
+    assert np.allclose(vals[:, :3], rec24["Pos"][ih[:ni.value] - 1], atol=5e-7)
